@@ -1,0 +1,96 @@
+"""Builds the runtime's shared libraries in-tree for gfx950.
+
+  lib/libgpumpi.so        host C-ABI (include/gpumpi/host.h): launcher, watchdog
+  lib/libgpumpi_progs.so  rank programs (tests, workloads, benchmarks) with a
+                          by-name registry, linked against libgpumpi.so
+
+Usage: python dphpc-gpu_mpi_amd/build.py [-j N] [--force]
+"""
+import argparse
+import concurrent.futures as cf
+import glob
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+OBJ = os.path.join(PKG, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+COMMON = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+    "-Wno-unused-result",
+]
+
+
+def _headers():
+    return glob.glob(os.path.join(ROOT, "include", "**", "*.h"), recursive=True) + \
+        glob.glob(os.path.join(CSRC, "**", "*.hpp"), recursive=True)
+
+
+def _stale(target, deps):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, obj, force):
+    deps = [src] + _headers()
+    if not force and not _stale(obj, deps):
+        return obj, "", 0.0
+    import time
+    t0 = time.time()
+    cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in (p.stdout + p.stderr).splitlines()
+                    if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"compile failed: {src}\n{err}")
+    return obj, err, time.time() - t0
+
+
+def build(jobs=8, force=False, verbose=True):
+    os.makedirs(LIB, exist_ok=True)
+    os.makedirs(OBJ, exist_ok=True)
+    host_src = [os.path.join(CSRC, "host", "launcher.cpp")]
+    prog_src = sorted(glob.glob(os.path.join(CSRC, "progs", "*.hip"))) + \
+        [os.path.join(CSRC, "progs", "registry.cpp")]
+    units = [(s, os.path.join(OBJ, os.path.basename(s) + ".o")) for s in host_src + prog_src]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_compile, s, o, force): s for s, o in units}
+        for f in cf.as_completed(futs):
+            obj, msg, dt = f.result()
+            if verbose and dt:
+                print(f"[build] {os.path.basename(futs[f])} {dt:.1f}s", flush=True)
+            if verbose and msg.strip():
+                print(msg, flush=True)
+    core = os.path.join(LIB, "libgpumpi.so")
+    host_objs = [o for s, o in units if s in host_src]
+    if force or _stale(core, host_objs):
+        subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", core] + host_objs,
+                       check=True)
+    progs = os.path.join(LIB, "libgpumpi_progs.so")
+    prog_objs = [o for s, o in units if s in prog_src]
+    if force or _stale(progs, prog_objs + [core]):
+        subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", progs] + prog_objs +
+                       ["-L", LIB, "-lgpumpi", "-Wl,-rpath,$ORIGIN"], check=True)
+    return core, progs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--force", action="store_true")
+    a = ap.parse_args()
+    build(a.j, a.force)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,120 @@
+// Helper-wave protocol.  Wave 0 (the rank's program) posts a Cmd in LDS and
+// bumps cmd_seq; waves 1..W-1 wake, run their share, and bump ack.  Only
+// LDS flags are used -- no s_barrier -- so the protocol is valid whatever
+// control flow of the user program wave 0 is in.
+#pragma once
+
+#include "copy.hpp"
+
+namespace gpumpi {
+namespace rt {
+
+__device__ void run_cmd(const Cmd& c, int t, int nt);  // coll.hpp
+
+// Run `c` on every wave of this rank (wave 0 included) and return when all
+// waves are done with it.
+__device__ inline void wg_run(const Cmd& c) {
+    RankLds* L = ctx();
+    const int waves = L->waves;
+    if (waves == 1) {
+        run_cmd(c, lane_id(), 64);
+        return;
+    }
+    const uint32_t seq = L->cmd_seq + 1;
+    if (elected()) L->cmd = c;
+    fence_release_wg();
+    if (elected()) __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    run_cmd(c, (int)threadIdx.x, waves * 64);
+    const uint32_t target = seq * (uint32_t)(waves - 1);
+    int n = 0;
+    while (__hip_atomic_load(&L->ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++n >= (1 << 20)) {
+            n = 0;
+            if (aborted()) break;
+        }
+    }
+}
+
+// Copy by this rank: wave 0 alone for small or private-destination copies,
+// every wave for large ones.
+__device__ inline void rank_copy(void* dst, const void* src, uint64_t n) {
+    if (n == 0) return;
+    if (is_private_ptr(dst)) {
+        copy_all_lanes(dst, src, n);
+        return;
+    }
+    if (is_private_ptr(src) || n < (uint64_t)ctx()->helper_min_bytes || ctx()->waves == 1) {
+        copy_part(dst, src, n, lane_id(), 64);
+        return;
+    }
+    Cmd c{};
+    c.op = CMD_COPY;
+    c.a[0] = (uint64_t)(uintptr_t)dst;
+    c.a[1] = (uint64_t)(uintptr_t)src;
+    c.a[2] = n;
+    wg_run(c);
+}
+
+// Body of waves 1..W-1.
+__device__ inline void helper_loop() {
+    RankLds* L = ctx();
+    uint32_t seen = 0;
+    const int period = 4096;
+    for (;;) {
+        uint32_t s;
+        int n = 0;
+        while ((s = __hip_atomic_load(&L->cmd_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == seen) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++n >= period) {
+                n = 0;
+                if (aborted()) return;
+            }
+        }
+        seen = s;
+        Cmd c = L->cmd;
+        if (c.op == CMD_EXIT) return;
+        run_cmd(c, (int)threadIdx.x, L->waves * 64);
+        fence_release_wg();
+        if (elected()) __hip_atomic_fetch_add(&L->ack, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// Wave 0 tells the helpers to leave (end of the rank's program).
+__device__ inline void helpers_exit() {
+    RankLds* L = ctx();
+    if (L->waves == 1) return;
+    const uint32_t seq = L->cmd_seq + 1;
+    if (elected()) L->cmd.op = CMD_EXIT;
+    fence_release_wg();
+    if (elected()) __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ---------------------------------------------------------------------------
+// Communicator barrier: a monotonically counting arrival word per
+// communicator; barrier k of a member completes when the word reaches
+// (k+1)*size.  Replaces the CUDA-internal grid sync of
+// gpu_libs/gpu_libc/cuda_mpi.cuh:519-521 and works on every communicator
+// (the reference's MPI_Barrier trapped on non-WORLD, mpi.cu:324-331).
+// ---------------------------------------------------------------------------
+__device__ inline bool comm_barrier(CommShared* c, int size) {
+    if (size <= 1) return true;
+    uint64_t old = 0;
+    fence_release_agent();
+    if (elected()) old = __hip_atomic_fetch_add(&c->bar, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    old = uniform_u64(old);
+    const uint64_t target = (old / (uint64_t)size + 1) * (uint64_t)size;
+    Spinner sp;
+    bool ok = true;
+    while (uniform_u64(ld_relaxed_agent(&c->bar)) < target) {
+        if (!sp.keep_going()) {
+            ok = false;
+            break;
+        }
+    }
+    fence_acquire_agent();
+    return ok;
+}
+
+}  // namespace rt
+}  // namespace gpumpi

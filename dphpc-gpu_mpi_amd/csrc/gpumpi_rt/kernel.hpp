@@ -1,0 +1,95 @@
+// The persistent rank kernel: one workgroup per MPI rank.
+//
+// Replaces __gpu_main_caller of gpu_libs/gpu_main/main.cu:102-130 (one
+// thread per rank there).  Wave 0 runs the program; waves 1..W-1 serve it.
+#pragma once
+
+namespace gpumpi {
+namespace rt {
+
+// World rank of workgroup `b`.  With xcd_map the dispatcher's round-robin
+// over XCDs (workgroup b runs on XCD b % X) is inverted so ranks
+// [k*P/X, (k+1)*P/X) share XCD k and its L2.
+__device__ __forceinline__ int rank_of_block(const Config& cfg, int b) {
+    const int P = cfg.nranks, X = cfg.num_xcd;
+    if (!cfg.xcd_map || X <= 1 || P % X) return b;
+    return (b % X) * (P / X) + b / X;
+}
+
+__device__ inline void rank_init(World* w) {
+    const Config& cfg = w->cfg;
+    const LdsLayout lay = lds_layout(cfg.nranks, cfg.max_requests);
+    // Zero the whole LDS image with every thread.
+    uint32_t* z = (uint32_t*)gpumpi_lds_base;
+    for (uint32_t i = threadIdx.x; i < lay.total / 4; i += blockDim.x) z[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        RankLds* L = ctx();
+        L->w = w;
+        L->rank = rank_of_block(cfg, (int)blockIdx.x);
+        L->nranks = cfg.nranks;
+        L->waves = cfg.waves;
+        L->threads = cfg.threads;
+        L->R = cfg.max_requests;
+        L->Q = cfg.chan_slots;
+        L->slot_bytes = cfg.slot_bytes;
+        L->eager_bytes = cfg.eager_bytes;
+        L->helper_min_bytes = cfg.helper_min_bytes;
+        L->spin_period = cfg.spin_abort_period;
+        L->off_reqs = lay.off_reqs;
+        L->off_ring = lay.off_ring;
+        L->off_rx = lay.off_rx;
+        L->off_tx = lay.off_tx;
+        L->off_unmatched = lay.off_unmatched;
+        L->off_freebits = lay.off_freebits;
+        L->lds_bytes = lay.total;
+    }
+    __syncthreads();
+}
+
+template <int (*Main)(int, char**)>
+__device__ inline void rank_body(World* w, int argc, char** argv) {
+    rank_init(w);
+    if (threadIdx.x < 64) {
+        const int ret = Main(argc, argv);
+        if (elected()) w->rank_ret[my_rank()] = ret;
+        helpers_exit();
+    } else {
+        helper_loop();
+    }
+}
+
+}  // namespace rt
+}  // namespace gpumpi
+
+// Kernel of a rank program: gpumpi_rank_kernel<prog> is what the host
+// launches (one workgroup per rank, dynamic LDS = lds_layout(P, R).total).
+template <int (*Main)(int, char**)>
+__global__ void __launch_bounds__(1024) gpumpi_rank_kernel(gpumpi::rt::World* w, int argc, char** argv) {
+    gpumpi::rt::rank_body<Main>(w, argc, argv);
+}
+
+// ---------------------------------------------------------------------------
+// Helpers for rank programs
+// ---------------------------------------------------------------------------
+// Per-rank globals of converted programs: `T var[MAX_RANKS]` indexed by the
+// rank (gpu_libs/gpu_main/global_vars.cuh:15 indexed by thread instead).
+#ifndef GPU_MPI_MAX_RANKS
+#define GPU_MPI_MAX_RANKS 1024
+#endif
+#define __gpu_global(var) (var[::gpumpi::rt::my_rank()])
+__device__ inline int __gpu_max_threads() { return GPU_MPI_MAX_RANKS; }
+
+// Rank heap, wave-uniform (every lane gets the same pointer).
+__device__ inline void* gpumpi_malloc(size_t bytes) { return gpumpi::rt::rank_malloc(bytes); }
+__device__ inline void gpumpi_free(void* p, size_t bytes) { gpumpi::rt::rank_free(p, bytes); }
+// The opaque argument block handed to the launch (host C-ABI user_args).
+template <typename T>
+__device__ inline T* gpumpi_user_args() { return (T*)gpumpi::rt::world()->user_args; }
+// This rank's world rank and the wall clock, without MPI calls.
+__device__ inline int gpumpi_world_rank() { return gpumpi::rt::my_rank(); }
+__device__ inline uint64_t gpumpi_ticks() { return gpumpi::rt::uniform_u64(gpumpi::rt::wall_ticks()); }
+// Run `f(t, nt)` on every thread of the rank (all waves), e.g. for local
+// data-parallel compute of a rank program.  `f` must be a device lambda
+// without captures of private (stack) memory.
+__device__ inline bool gpumpi_is_leader_lane() { return gpumpi::rt::elected(); }

@@ -1,0 +1,461 @@
+// Point-to-point engine.
+//
+// Replaces the reference's lock-based matching engine
+// (gpu_libs/gpu_libc/cuda_mpi.cu:65-471: isend/irecv, progressStarted/
+// Posted/Completed Send/Recv, test, wait) with lock-free per-(src,dst)
+// channels in HBM:
+//
+//   * a channel is a ring of Q slots written only by the sender; a slot's
+//     envelope word (sequence, context, tag) is stored last with release
+//     semantics, so a receiver that sees the sequence sees the slot;
+//   * messages up to the eager size travel inside the slot (the send
+//     completes at once); larger ones carry the sender's buffer address and
+//     the receiver copies straight from it, then releases the sender's
+//     completion word (rendezvous);
+//   * the receiver matches: it scans the slots of one channel (or of every
+//     member's channel for MPI_ANY_SOURCE) with one lane per slot and a
+//     wavefront ballot, taking the oldest matching message -- MPI's
+//     non-overtaking order per (source, context, tag);
+//   * the receiver publishes how far it has consumed; the sender reuses a
+//     slot only after that.
+//
+// Rank-private bookkeeping (requests, channel cursors) lives in LDS.
+#pragma once
+
+#include "comm.hpp"
+
+namespace gpumpi {
+namespace rt {
+
+__device__ __forceinline__ uint8_t* slot_at(int dst, int src, uint32_t seq) {
+    RankLds* L = ctx();
+    const uint64_t idx = ((uint64_t)dst * (uint64_t)L->nranks + (uint64_t)src) * (uint64_t)L->Q +
+                         (uint64_t)(seq & (uint32_t)(L->Q - 1));
+    return world()->slots + idx * (uint64_t)L->slot_bytes;
+}
+
+// ---------------------------------------------------------------------------
+// Request pool (LDS).  Handles are (stamp << 16) | (index + 1) so a stale
+// handle is detected.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ MPI_Request req_handle(int idx) {
+    return (MPI_Request)(uintptr_t)(((uint64_t)reqs()[idx].stamp << 16) | (uint64_t)(idx + 1));
+}
+__device__ __forceinline__ int req_index(MPI_Request h) {
+    const uint64_t v = (uint64_t)(uintptr_t)h;
+    const int idx = (int)(v & 0xffff) - 1;
+    if (idx < 0 || idx >= ctx()->R) return -1;
+    if (reqs()[idx].kind == REQ_FREE || reqs()[idx].stamp != (uint32_t)(v >> 16)) return -1;
+    return idx;
+}
+
+// Ring entry i (absolute position) is live iff its request is in use and
+// was pushed at i; entries of freed (and possibly reused) requests are skipped.
+__device__ __forceinline__ bool ring_live(int i, int* idx_out) {
+    const int idx = ring()[i % ctx()->R];
+    *idx_out = idx;
+    const Req* q = &reqs()[idx];
+    return q->kind != REQ_FREE && q->ringpos == i;
+}
+
+__device__ inline void ring_pop_dead() {
+    RankLds* L = ctx();
+    int idx;
+    while (L->ring_head < L->ring_tail && !ring_live(L->ring_head, &idx)) L->ring_head++;
+}
+
+// Squeeze dead entries out of the ring, keeping posting order.
+__device__ inline void ring_compact() {
+    RankLds* L = ctx();
+    const int R = L->R;
+    int out = L->ring_head;
+    for (int i = L->ring_head; i < L->ring_tail; i++) {
+        int idx;
+        if (ring_live(i, &idx)) {
+            ring()[out % R] = (uint16_t)idx;
+            reqs()[idx].ringpos = out;
+            out++;
+        }
+    }
+    L->ring_tail = out;
+}
+
+__device__ inline void progress();
+
+__device__ inline int req_alloc() {
+    RankLds* L = ctx();
+    const int R = L->R;
+    const int nwords = (R + 63) / 64;
+    for (int attempt = 0;; attempt++) {
+        ring_pop_dead();
+        if (L->ring_tail - L->ring_head >= R) ring_compact();
+        if (L->ring_tail - L->ring_head < R) {
+            for (int wi = 0; wi < nwords; wi++) {
+                uint64_t bits = free_bits()[wi];
+                if (wi == nwords - 1 && (R & 63)) bits |= ~((1ull << (R & 63)) - 1);
+                if (bits != ~0ull) {
+                    const int b = __builtin_ctzll(~bits);
+                    const int idx = wi * 64 + b;
+                    free_bits()[wi] = free_bits()[wi] | (1ull << b);
+                    Req* q = &reqs()[idx];
+                    q->state = RS_PENDING;
+                    q->flags = 0;
+                    uint32_t st = L->stamp + 1;
+                    if (st == 0) st = 1;
+                    L->stamp = st;
+                    q->stamp = st;
+                    q->ringpos = L->ring_tail;
+                    ring()[L->ring_tail % R] = (uint16_t)idx;
+                    L->ring_tail++;
+                    return idx;
+                }
+            }
+        }
+        if (attempt > 64 || aborted()) break;
+        progress();
+    }
+    GPUMPI_FAIL(MPI_ERR_REQUEST, "request pool exhausted (raise pending requests, -e)");
+    return -1;
+}
+
+__device__ __forceinline__ void req_release(int idx) {
+    Req* q = &reqs()[idx];
+    q->kind = REQ_FREE;
+    free_bits()[idx >> 6] = free_bits()[idx >> 6] & ~(1ull << (idx & 63));
+}
+
+// ---------------------------------------------------------------------------
+// Staging (bounce) buffers for private / LDS buffers.  The first half of the
+// rank's staging slice backs rendezvous sends, the second half collectives.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint8_t* staging_base() {
+    World* w = world();
+    return w->staging + (uint64_t)my_rank() * w->cfg.staging_bytes;
+}
+__device__ __forceinline__ uint64_t staging_half() { return world()->cfg.staging_bytes / 2; }
+
+__device__ inline uint8_t* bounce_alloc(uint64_t bytes) {
+    RankLds* L = ctx();
+    const uint64_t need = (bytes + 255) & ~(uint64_t)255;
+    if (L->bounce_users == 0) L->bounce_top = 0;
+    if (L->bounce_top + need > staging_half()) {
+        GPUMPI_FAIL(MPI_ERR_NO_MEM, "staging space for a private-buffer send exhausted");
+        return nullptr;
+    }
+    uint8_t* p = staging_base() + L->bounce_top;
+    L->bounce_top += need;
+    L->bounce_users++;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Send side
+// ---------------------------------------------------------------------------
+__device__ inline bool try_post_send(int idx) {
+    RankLds* L = ctx();
+    Req* q = &reqs()[idx];
+    const int dst = q->peer, me = L->rank;
+    TxChan* tc = &txc()[dst];
+    const uint32_t s = tc->seq;
+    const uint32_t Q = (uint32_t)L->Q;
+    if (s - tc->head_cache >= Q) {
+        const uint32_t h = uniform_u32(ld_acquire_agent(&world()->chan_head[(uint64_t)dst * L->nranks + me]));
+        tc->head_cache = h;
+        if (s - h >= Q) return false;
+    }
+    uint8_t* slot = slot_at(dst, me, s);
+    SlotHdr* hdr = (SlotHdr*)slot;
+    const bool eager = (q->bytes <= (uint64_t)L->eager_bytes) && !(q->flags & RF_SYNC);
+    if (eager) {
+        copy_part(slot + kSlotHeaderBytes, (const void*)(uintptr_t)q->buf, q->bytes, lane_id(), 64);
+    }
+    if (elected()) {
+        hdr->bytes = q->bytes;
+        hdr->flags = eager ? SLOT_EAGER : SLOT_RNDV;
+        hdr->src_world = (uint32_t)me;
+        if (!eager) {
+            hdr->src_addr = q->buf;
+            hdr->done_addr = (uint64_t)(uintptr_t)&world()->done_words[(uint64_t)me * L->R + idx];
+            hdr->done_val = q->stamp;
+        }
+    }
+    // Envelope last: the release fence orders the payload and header (every
+    // lane's stores) before it.
+    fence_release_agent();
+    if (elected()) st_relaxed_agent(&hdr->env, pack_env(s + 1, q->ctx, q->tag));
+    tc->seq = s + 1;
+    q->state = eager ? RS_COMPLETE : RS_WAIT_DONE;
+    return true;
+}
+
+// Start a send of `bytes` at `buf` to world rank `dst`.  Returns the request
+// index, or -1 after an error.
+__device__ inline int start_send(const void* buf, uint64_t bytes, int dst, int tag, int ctxid,
+                                 MPI_Comm comm, bool sync) {
+    RankLds* L = ctx();
+    const int idx = req_alloc();
+    if (idx < 0) return -1;
+    Req* q = &reqs()[idx];
+    q->kind = REQ_SEND;
+    q->state = RS_PENDING;
+    q->flags = sync ? RF_SYNC : 0;
+    q->peer = dst;
+    q->tag = tag;
+    q->ctx = ctxid;
+    q->bytes = bytes;
+    q->buf = (uint64_t)(uintptr_t)buf;
+    q->comm = (uint64_t)(uintptr_t)comm;
+    q->st_error = MPI_SUCCESS;
+    if (dst == MPI_PROC_NULL) {
+        q->state = RS_COMPLETE;
+        return idx;
+    }
+    const bool eager = bytes <= (uint64_t)L->eager_bytes && !sync;
+    if (!eager && !is_global_ptr(buf)) {
+        uint8_t* b = bounce_alloc(bytes);
+        if (!b) {
+            q->state = RS_COMPLETE;
+            q->st_error = MPI_ERR_NO_MEM;
+            return idx;
+        }
+        copy_part(b, buf, bytes, lane_id(), 64);
+        q->buf = (uint64_t)(uintptr_t)b;
+        q->flags |= RF_STAGED;
+    }
+    // Non-overtaking: a send may not take a slot while an earlier send to
+    // the same destination is still waiting for one.
+    bool earlier_waiting = false;
+    if (L->pending_sends > 0) {
+        for (int i = L->ring_head; i < L->ring_tail && !earlier_waiting; i++) {
+            int oi;
+            if (!ring_live(i, &oi) || oi == idx) continue;
+            const Req* o = &reqs()[oi];
+            if (o->kind == REQ_SEND && o->state == RS_PENDING && o->peer == dst) earlier_waiting = true;
+        }
+    }
+    if (earlier_waiting || !try_post_send(idx)) L->pending_sends++;
+    return idx;
+}
+
+// ---------------------------------------------------------------------------
+// Receive side
+// ---------------------------------------------------------------------------
+// Deliver message `seq` of channel src->me to receive request `idx`.
+__device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
+    RankLds* L = ctx();
+    Req* q = &reqs()[idx];
+    const int me = L->rank;
+    fence_acquire_agent();
+    SlotHdr* hdr = (SlotHdr*)slot_at(me, src, seq);
+    const uint64_t bytes = uniform_u64(hdr->bytes);
+    const uint32_t flags = uniform_u32(hdr->flags);
+    uint64_t n = bytes;
+    int err = MPI_SUCCESS;
+    if (bytes > q->bytes) {
+        n = q->bytes;
+        err = MPI_ERR_TRUNCATE;
+    }
+    if (flags & SLOT_EAGER) {
+        void* dst = (void*)(uintptr_t)q->buf;
+        if (is_private_ptr(dst))
+            copy_all_lanes(dst, (uint8_t*)hdr + kSlotHeaderBytes, n);
+        else
+            copy_part(dst, (uint8_t*)hdr + kSlotHeaderBytes, n, lane_id(), 64);
+    } else {
+        const void* from = (const void*)(uintptr_t)uniform_u64(hdr->src_addr);
+        uint32_t* done = (uint32_t*)(uintptr_t)uniform_u64(hdr->done_addr);
+        const uint32_t dval = uniform_u32(hdr->done_val);
+        rank_copy((void*)(uintptr_t)q->buf, from, n);
+        fence_release_agent();
+        if (elected()) st_relaxed_agent(done, dval);
+    }
+    // Mark consumed; publish the new head once the oldest slots are done.
+    RxChan* rc = &rxc()[src];
+    const uint32_t h0 = rc->head;
+    uint64_t mask = rc->mask | (1ull << (seq - h0));
+    uint32_t h = h0;
+    while (mask & 1ull) {
+        mask >>= 1;
+        h++;
+    }
+    rc->mask = mask;
+    rc->head = h;
+    if (h != h0) {
+        fence_release_agent();
+        if (elected()) st_relaxed_agent(&world()->chan_head[(uint64_t)me * L->nranks + src], h);
+    }
+    q->state = RS_COMPLETE;
+    q->got = n;
+    q->st_source = src;
+    q->st_tag = env_tag(env);
+    q->st_error = err;
+    if (q->peer < 0) L->unmatched_any--;
+    else unmatched_src()[q->peer] = (uint16_t)(unmatched_src()[q->peer] - 1);
+    if (err != MPI_SUCCESS) GPUMPI_FAIL(err, "message truncated: receive buffer too small");
+}
+
+// Look for the oldest message of channel src->me matching request idx.
+__device__ inline bool match_from(int idx, int src) {
+    RankLds* L = ctx();
+    const Req* q = &reqs()[idx];
+    const int Q = L->Q;
+    const RxChan rc = rxc()[src];
+    const int j = lane_id();
+    bool ok = false;
+    uint64_t env = 0;
+    if (j < Q && !((rc.mask >> j) & 1ull)) {
+        env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, rc.head + (uint32_t)j))->env);
+        ok = env_seq(env) == rc.head + (uint32_t)j + 1u && env_ctx(env) == q->ctx &&
+             (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
+    }
+    const uint64_t b = ballot(ok);
+    if (!b) return false;
+    const int first = __builtin_ctzll(b);
+    env = uniform_u64(__shfl(env, first));
+    deliver(idx, src, rc.head + (uint32_t)first, env);
+    return true;
+}
+
+// MPI_ANY_SOURCE: scan the channels of every member of the communicator,
+// lowest member first.
+__device__ inline bool match_any(int idx) {
+    RankLds* L = ctx();
+    const Req* q = &reqs()[idx];
+    CommShared* c = comm_get((MPI_Comm)(uintptr_t)q->comm);
+    const int size = c->size;
+    const int Q = L->Q;
+    const int total = size * Q;
+    for (int base = 0; base < total; base += 64) {
+        const int k = base + lane_id();
+        bool ok = false;
+        uint64_t env = 0;
+        int src = 0;
+        uint32_t seq = 0;
+        if (k < total) {
+            const int r = k / Q, j = k % Q;
+            src = comm_world_of(c, r);
+            const RxChan rc = rxc()[src];
+            seq = rc.head + (uint32_t)j;
+            if (!((rc.mask >> j) & 1ull)) {
+                env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, seq))->env);
+                ok = env_seq(env) == seq + 1u && env_ctx(env) == q->ctx &&
+                     (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
+            }
+        }
+        const uint64_t b = ballot(ok);
+        if (b) {
+            const int first = __builtin_ctzll(b);
+            env = uniform_u64(__shfl(env, first));
+            src = uniform_i32(__shfl(src, first));
+            seq = uniform_u32(__shfl((int)seq, first));
+            deliver(idx, src, seq, env);
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool try_match(int idx) {
+    const Req* q = &reqs()[idx];
+    return q->peer >= 0 ? match_from(idx, q->peer) : match_any(idx);
+}
+
+// One pass over every active request in posting order.
+__device__ inline void progress() {
+    RankLds* L = ctx();
+    const int R = L->R;
+    uint64_t blocked = 0;  // destinations (hashed) whose earlier send found no slot
+    for (int i = L->ring_head; i < L->ring_tail; i++) {
+        int idx;
+        if (!ring_live(i, &idx)) continue;
+        Req* q = &reqs()[idx];
+        if (q->kind == REQ_SEND) {
+            if (q->state == RS_PENDING) {
+                const uint64_t hb = 1ull << (q->peer & 63);
+                if (blocked & hb) continue;
+                if (try_post_send(idx)) L->pending_sends--;
+                else blocked |= hb;
+            }
+            if (q->state == RS_WAIT_DONE) {
+                const uint32_t v = uniform_u32(
+                    ld_relaxed_agent(&world()->done_words[(uint64_t)L->rank * R + idx]));
+                if (v == q->stamp) {
+                    fence_acquire_agent();
+                    q->state = RS_COMPLETE;
+                    if (q->flags & RF_STAGED) L->bounce_users--;
+                }
+            }
+        } else if (q->kind == REQ_RECV && q->state == RS_PENDING) {
+            try_match(idx);
+        }
+    }
+}
+
+// Start a receive of up to `bytes` into `buf` from world rank `src`
+// (or -1 = any member of comm).
+__device__ inline int start_recv(void* buf, uint64_t bytes, int src, int tag, int ctxid, MPI_Comm comm) {
+    RankLds* L = ctx();
+    const int idx = req_alloc();
+    if (idx < 0) return -1;
+    Req* q = &reqs()[idx];
+    q->kind = REQ_RECV;
+    q->state = RS_PENDING;
+    q->flags = 0;
+    q->peer = src;
+    q->tag = tag;
+    q->ctx = ctxid;
+    q->bytes = bytes;
+    q->buf = (uint64_t)(uintptr_t)buf;
+    q->comm = (uint64_t)(uintptr_t)comm;
+    q->got = 0;
+    q->st_error = MPI_SUCCESS;
+    if (src == MPI_PROC_NULL) {
+        q->state = RS_COMPLETE;
+        q->st_source = MPI_PROC_NULL;
+        q->st_tag = MPI_ANY_TAG;
+        return idx;
+    }
+    // An earlier unmatched receive that could take the same message must get
+    // the first chance at it (MPI ordering of posted receives).
+    const bool earlier = L->unmatched_any > 0 || (src < 0 ? false : unmatched_src()[src] > 0) ||
+                         (src < 0 && L->ring_tail - L->ring_head > 1);
+    if (src < 0) L->unmatched_any++;
+    else unmatched_src()[src] = (uint16_t)(unmatched_src()[src] + 1);
+    if (earlier) progress();
+    else try_match(idx);
+    return idx;
+}
+
+// Status of a completed request (source translated to a rank of its comm).
+__device__ inline void fill_status(int idx, MPI_Status* st) {
+    if (!st) return;
+    const Req* q = &reqs()[idx];
+    int src = q->st_source;
+    if (q->kind == REQ_RECV && src >= 0) src = comm_rank_of_world(comm_get((MPI_Comm)(uintptr_t)q->comm), src);
+    st->MPI_SOURCE = q->kind == REQ_RECV ? src : MPI_ANY_SOURCE;
+    st->MPI_TAG = q->kind == REQ_RECV ? q->st_tag : MPI_ANY_TAG;
+    st->MPI_ERROR = q->st_error;
+    st->gpumpi_count_lo = (int)(uint32_t)q->got;
+    st->gpumpi_count_hi = (int)(uint32_t)(q->got >> 32);
+}
+
+// Wait for request idx; fills the status and frees the request.
+__device__ inline int wait_req(int idx, MPI_Status* st) {
+    Req* q = &reqs()[idx];
+    Spinner sp;
+    while (q->state != RS_COMPLETE) {
+        progress();
+        if (q->state == RS_COMPLETE) break;
+        if (!sp.keep_going()) {
+            q->st_error = MPI_ERR_OTHER;
+            break;
+        }
+    }
+    fill_status(idx, st);
+    const int err = q->st_error;
+    req_release(idx);
+    return err;
+}
+
+}  // namespace rt
+}  // namespace gpumpi

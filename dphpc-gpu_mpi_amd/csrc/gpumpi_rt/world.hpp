@@ -1,0 +1,119 @@
+// World state shared by the host launcher and the device runtime.
+//
+// Everything here lives in HBM (one allocation set per launch shape) except
+// the abort word and the error record, which are pinned host memory so a
+// host watchdog can stop a running kernel and read why it stopped.
+//
+// Replaces the reference's managed-memory SharedState
+// (gpu_libs/gpu_libc/cuda_mpi.cuh:459-533) and per-thread
+// ThreadPrivateState (cuda_mpi.cuh:254-370): rank-private state moves into
+// the rank workgroup's LDS (see lds.hpp), rank-to-rank state becomes
+// per-(src,dst) channels below.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gpumpi {
+namespace rt {
+
+constexpr int kMaxWaves = 16;        // 1024-thread workgroup
+constexpr int kSlotHeaderBytes = 64; // channel slot header
+constexpr int kMaxChanSlots = 64;    // consumed-mask width per channel
+constexpr int kMaxCommSlots = 8;     // words of per-member collective slot
+
+// Launch-time configuration (copied into the World in HBM).
+struct Config {
+    int nranks;           // MPI world size = workgroups
+    int threads;          // threads per rank workgroup (multiple of 64)
+    int waves;            // threads / 64
+    int chan_slots;       // slots per (src,dst) channel, power of two <= 64
+    int slot_bytes;       // header + eager payload, multiple of 64
+    int eager_bytes;      // slot_bytes - header
+    int max_requests;     // request pool per rank (LDS)
+    int helper_min_bytes; // copies at least this large use every wave
+    uint64_t staging_bytes; // per-rank bounce buffer for private/LDS buffers
+    uint64_t arena_bytes;   // per-rank device heap (gpumpi_malloc)
+    uint64_t wall_hz;       // s_memrealtime frequency
+    int xcd_map;          // 1: consecutive ranks share an XCD
+    int num_xcd;          // XCDs the dispatcher round-robins over
+    uint64_t comm_pool_bytes;
+    int lds_bytes;        // dynamic LDS per workgroup
+    int spin_abort_period; // polls between abort-word checks
+};
+
+// Envelope word of a slot: [63:32] sequence+1, [31:16] context, [15:0] tag.
+// Written last, with release semantics, by the sender.
+__host__ __device__ inline uint64_t pack_env(uint32_t seq1, int ctx, int tag) {
+    return ((uint64_t)seq1 << 32) | ((uint64_t)(uint16_t)ctx << 16) | (uint64_t)(uint16_t)tag;
+}
+__host__ __device__ inline uint32_t env_seq(uint64_t e) { return (uint32_t)(e >> 32); }
+__host__ __device__ inline int env_ctx(uint64_t e) { return (int)(int16_t)(uint16_t)(e >> 16); }
+__host__ __device__ inline int env_tag(uint64_t e) { return (int)(int16_t)(uint16_t)e; }
+
+enum SlotFlags : uint32_t { SLOT_EAGER = 1u, SLOT_RNDV = 2u };
+
+// One message slot of a (src,dst) channel; the eager payload follows the
+// header inside the slot (slot_bytes - 64 bytes of it).
+struct alignas(64) SlotHdr {
+    uint64_t env;       // see pack_env
+    uint64_t bytes;     // message length
+    uint64_t src_addr;  // rendezvous: sender's buffer
+    uint64_t done_addr; // rendezvous: sender's completion word
+    uint32_t done_val;  // value the receiver stores there
+    uint32_t flags;     // SlotFlags
+    uint32_t src_world; // sender's world rank
+    uint32_t pad0;
+    uint64_t pad1[2];
+};
+static_assert(sizeof(SlotHdr) == kSlotHeaderBytes, "slot header is one 64B line");
+
+// Per-member entry of a communicator's collective table.
+struct alignas(64) CollSlot {
+    uint64_t w[kMaxCommSlots];
+};
+
+// Communicator state shared by all members (HBM).  MPI_Comm handles other
+// than the predefined constants point at one of these.
+struct alignas(128) CommShared {
+    uint64_t bar;        // monotonically counting barrier (own 128B line)
+    uint64_t pad0[15];
+    int ctx;             // context id carried by every message
+    int size;
+    int kind;            // 0 world, 1 self, 2 derived
+    int pad1;
+    int* world_of;       // [size]: comm rank -> world rank
+    int* rank_of_world;  // [world size]: world rank -> comm rank or -1
+    CollSlot* slots;     // [size]: collective pointer table
+};
+
+struct ErrorRecord {
+    int code;      // first error's MPI error code
+    int rank;      // world rank that raised it
+    int line;      // source line in the runtime
+    int count;     // number of errors raised
+    char msg[112];
+};
+
+struct World {
+    Config cfg;
+    uint8_t* slots;        // [P*P*Q] slots of slot_bytes, index (dst*P+src)*Q+k
+    uint32_t* chan_head;   // [P*P] receiver-published consumed count, dst*P+src
+    uint32_t* done_words;  // [P*R] rendezvous completion words
+    uint8_t* staging;      // [P*staging_bytes]
+    uint8_t* arena;        // [P*arena_bytes]
+    uint64_t* arena_top;   // [P] bump offsets
+    CommShared* comm_world;
+    CommShared* comm_self; // [P]
+    uint8_t* comm_pool;    // bump region for derived communicators
+    uint64_t* comm_pool_top;
+    int* next_ctx;
+    int* rank_ret;         // [P] return value of each rank's program
+    uint64_t* rank_stats;  // [P*8] per-rank counters
+    int* abort_word;       // pinned host
+    ErrorRecord* err;      // pinned host
+    void* user_args;       // program arguments (opaque)
+};
+
+}  // namespace rt
+}  // namespace gpumpi

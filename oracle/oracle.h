@@ -1,0 +1,107 @@
+/*
+ * CPU oracle for the device-resident MPI hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it restates, in
+ * plain C, the arithmetic and data movement the reference MPI runtime
+ * (lilyyangyn/dphpc-gpu_mpi) performs on the hot path, so the HIP runtime
+ * can be compared against it on the same inputs.  Only tests/, the smoke
+ * entry point and bench.py's CPU-baseline leg may load it.  The product
+ * path never calls into this file.
+ *
+ * Each function cites the reference file:line it follows.  How each one is
+ * pinned (golden vectors, closed forms, host MPI) is listed next to it; a
+ * function marked "parity unpinned" has no independent check.
+ */
+#ifndef GPUMPI_ORACLE_H
+#define GPUMPI_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- datatypes (gpu_libs/gpu_mpi/mpi_types_list.cuh:3-33) ------------- */
+/* Enum order and element sizes; MPI_LONG_DOUBLE is an 8-byte double and the
+ * long-double complex a 16-byte double complex, as in the reference. */
+int orc_type_size(int dtype);
+/* 1 if (dtype, op) is a legal reduction pair (operators.cu:153-200). */
+int orc_op_allowed(int dtype, int op);
+
+/* ---- reductions (gpu_libs/gpu_mpi/mpi.cu:188-234, operators.cu:73-218) --
+ * Generic Reduce: acc = x_0, then acc[i] = op(x_r[i], acc[i]) for r=1..P-1
+ * in rank order.  `inputs` holds P contiguous vectors of `count` elements.
+ * Returns 0, or -1 if the pair is illegal.
+ * Pinned by: closed forms (sum of (r+1)(i+1)), the int16 PROD wraparound
+ * known-answer test (tests/collectives_test.cu:6-54), and MPICH output for
+ * every legal (type, op) pair on integer-valued inputs (tests/golden). */
+int orc_reduce_fold(int dtype, int op, int nranks, size_t count,
+                    const void* inputs, void* out);
+
+/* ---- pi by the midpoint rule (toolchain_tests/pi/cpi.c:41-56) ----------
+ * Partial sum of rank `rank` of `nranks` for n intervals (cyclic split
+ * i = rank+1, rank+1+nranks, ...), then the rank-order fold of the partials.
+ * Pinned by: closed form pi (cpi.c:23) and the reference cpi.c itself run
+ * under host MPI. */
+double orc_pi_partial(int n, int rank, int nranks);
+double orc_pi_fold(int n, int nranks, double* partials_out);
+
+/* ---- seeded payloads ----------------------------------------------------
+ * Counter-based 64-bit words: word i of rank r under seed s.  Bytes are the
+ * little-endian bytes of consecutive words.  Pinned by: splitmix64's
+ * published test vector (seed 1234567). */
+uint64_t orc_mix64(uint64_t x);
+uint64_t orc_payload_word(uint64_t seed, uint64_t rank, uint64_t i);
+void orc_payload_bytes(uint64_t seed, uint64_t rank, size_t nbytes, uint8_t* out);
+uint64_t orc_splitmix64_next(uint64_t* state);
+
+/* ---- NPB IS (toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI/IS/is.c) ----------
+ * randlc (is.c:408-476), find_my_seed (:489-533), create_seq (:535-553),
+ * rank (:631-930), full_verify (:561-625), main loop (:1060-1137).
+ * Pinned by: NPB's own known-answer arrays (is.c:254-282, :869-913) and the
+ * full-verify pass count 5*MAX_ITERATIONS + comm_size (is.c:1137). */
+double orc_randlc(double* x, double a);
+double orc_find_my_seed(int kn, int np, long nn, double s, double a);
+
+typedef struct orc_is_result {
+    int passed_verification;  /* summed over ranks, as in is.c:1113-1137 */
+    int full_verify_errors;   /* keys out of order, summed over ranks */
+    int64_t total_keys_after; /* sum of total_local_keys over ranks */
+    uint64_t sorted_checksum; /* sum over ranks of sum_i mix64(key_i + i*2^32) */
+} orc_is_result;
+
+/* Runs the whole IS benchmark (one untimed + 10 iterations + full_verify)
+ * for `cls` in {'S','W','A','B'} on `nranks` simulated ranks.  If
+ * `keys_out` is non-NULL it receives the globally sorted keys (TOTAL_KEYS).
+ * If bucket_totals_out is non-NULL it receives bucket_size_totals of the
+ * last iteration (NUM_BUCKETS + 5 ints). */
+int orc_is_run(char cls, int nranks, orc_is_result* res, int32_t* keys_out,
+               int32_t* bucket_totals_out);
+/* Keys of rank `rank` right after create_seq (num_keys of them). */
+int orc_is_keys(char cls, int nranks, int rank, int32_t* out);
+
+/* ---- compressed wire codecs (BF16, OCP FP8 e4m3 with per-block scale) --
+ * Written from the published formats: BF16 = top half of binary32 with
+ * round-to-nearest-even; FP8 e4m3 (OCP "fn": bias 7, max 448, no inf,
+ * S.1111.111 = NaN), saturating, with an FP32 scale amax/448 per block.
+ * Pinned by: torch's bfloat16 / float8_e4m3fn casts on CPU. */
+uint16_t orc_f32_to_bf16(float x);
+float orc_bf16_to_f32(uint16_t h);
+uint8_t orc_f32_to_e4m3(float x);
+float orc_e4m3_to_f32(uint8_t q);
+/* Quantise n floats in blocks of `block`: q[n] codes and scales[ceil(n/block)]. */
+void orc_e4m3_block_encode(const float* x, size_t n, int block, uint8_t* q, float* scales);
+void orc_e4m3_block_decode(const uint8_t* q, const float* scales, size_t n, int block, float* y);
+
+/* ---- communicator split (gpu_libs/gpu_mpi/communicator.cu:147-213) ----
+ * For each rank: new rank = position in its colour's member list ordered by
+ * (key, parent rank); size = members of that colour.  Pinned by the
+ * known-answer table of tests/communicator_test.cu:93-152. */
+void orc_comm_split(int nranks, const int* colors, const int* keys,
+                    int* new_rank, int* new_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,46 @@
+"""Regenerates the host-MPI golden fixtures in this directory.
+
+Runs in the build container (it needs host MPICH at /opt/conda/bin and, for
+the pi fixture, the reference's cpi.c under /root/reference):
+
+  * reduce_mpich.bin -- oracle/hostmpi/reduce_golden.c under `mpirun -np 5`:
+    rank 0's MPI_Reduce result for every legal (datatype, op) pair.
+  * cpi_mpich.json   -- the reference's toolchain_tests/pi/cpi.c compiled with
+    host mpicc (oracle/Makefile `ref`) and run with N=1e8 at several rank
+    counts; the printed pi is recorded.
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+ORACLE = os.path.join(ROOT, "oracle")
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+def run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    return subprocess.run(cmd, check=True, **kw)
+
+
+def main():
+    run(["make", "-C", ORACLE, "_build/hostmpi_reduce_golden", "ref"])
+    run([MPIRUN, "-np", "5", os.path.join(ORACLE, "_build", "hostmpi_reduce_golden"),
+         os.path.join(HERE, "reduce_mpich.bin")])
+    pis = {}
+    for np_ in (1, 8, 64):
+        out = run([MPIRUN, "-np", str(np_), os.path.join(ORACLE, "_ref", "cpi"), "100000000"],
+                  capture_output=True, text=True).stdout
+        m = re.search(r"pi is approximately ([0-9.]+)", out)
+        pis[str(np_)] = m.group(1)
+    with open(os.path.join(HERE, "cpi_mpich.json"), "w") as f:
+        json.dump({"n": 100000000, "pi_printed": pis,
+                   "source": "toolchain_tests/pi/cpi.c under MPICH 3.3.2"}, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

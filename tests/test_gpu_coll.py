@@ -1,0 +1,211 @@
+"""Collective parity against the CPU oracle: every legal (datatype, op)
+reduction bit-exact with the rank-order fold, Bcast / Alltoallv / Gather(v) /
+Scatter(v) / Allgather(v) / Alltoall payloads exact, and the reference's
+cpi.c as a rank program (partials bit-exact, pi within 1e-13 of the FP64
+oracle and of host MPI)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rand_inputs(orc, dtype, P, n, rng):
+    npdt = np.dtype(orc.NP_DTYPE[dtype])
+    if dtype == "MPI_C_BOOL":
+        return rng.integers(0, 2, (P, n)).astype(np.uint8)
+    if npdt.kind in "iu":
+        info = np.iinfo(npdt)
+        return rng.integers(info.min, info.max, (P, n), dtype=npdt, endpoint=True)
+    if npdt.kind == "f":
+        return (rng.standard_normal((P, n)) * 1e3).astype(npdt)
+    re = rng.standard_normal((P, n))
+    im = rng.standard_normal((P, n))
+    return (re + 1j * im).astype(npdt)
+
+
+def _reduce(gm, orc, dtype, op, x, root=-1, comm_mode=0, ncolors=1, in_place=False, threads=256):
+    P, n = x.shape
+    es = x.dtype.itemsize
+    stride = max(256, (n * es + 255) // 256 * 256)
+    ib = gm.DeviceBuffer(stride * P)
+    ob = gm.DeviceBuffer(stride * P)
+    ob.zero()
+    for r in range(P):
+        ib.upload(x[r], offset=r * stride)
+        if in_place:
+            ob.upload(x[r], offset=r * stride)
+    res = gm.run("coll_reduce", P, [0, orc.DT[dtype], orc.OP[op], n, ib, ob, stride, root, comm_mode,
+                                    ncolors, 1, 0, 1 if in_place else 0], threads_per_rank=threads,
+                 timeout_s=60.0)
+    assert res.status == 0, res.describe()
+    return [ob.download(x.dtype, n, offset=r * stride) for r in range(P)]
+
+
+def _legal_pairs(orc):
+    return [(t, o) for t in orc.DTYPES for o in orc.OPS if orc.op_allowed(t, o)]
+
+
+def test_allreduce_every_legal_pair(gm, orc):
+    rng = np.random.default_rng(11)
+    pairs = _legal_pairs(orc)
+    assert len(pairs) == 216
+    P, n = 5, 37
+    for dtype, op in pairs:
+        print(f"[gpu] allreduce {dtype} {op}", flush=True)
+        x = _rand_inputs(orc, dtype, P, n, rng)
+        exp = orc.reduce_fold(dtype, op, x)
+        outs = _reduce(gm, orc, dtype, op, x, threads=64)
+        for r in range(P):
+            assert outs[r].tobytes() == exp.tobytes(), (dtype, op, r)
+
+
+@pytest.mark.parametrize("dtype,op", [("MPI_DOUBLE", "MPI_SUM"), ("MPI_FLOAT", "MPI_SUM"), ("MPI_INT", "MPI_SUM"),
+                                      ("MPI_LONG", "MPI_MAX"), ("MPI_UINT8_T", "MPI_BXOR"),
+                                      ("MPI_C_DOUBLE_COMPLEX", "MPI_PROD")])
+@pytest.mark.parametrize("P,n", [(7, 1), (16, 1029), (64, 100003), (128, 4116 // 4)])
+def test_allreduce_sizes(gm, orc, dtype, op, P, n):
+    rng = np.random.default_rng(P * 1000 + n)
+    x = _rand_inputs(orc, dtype, P, n, rng)
+    exp = orc.reduce_fold(dtype, op, x)
+    print(f"[gpu] allreduce {dtype} {op} P={P} n={n}", flush=True)
+    outs = _reduce(gm, orc, dtype, op, x)
+    for r in range(P):
+        assert outs[r].tobytes() == exp.tobytes(), r
+
+
+@pytest.mark.parametrize("root", [0, 3])
+def test_reduce_root_and_in_place(gm, orc, root):
+    rng = np.random.default_rng(5)
+    P, n = 6, 1000
+    x = (rng.standard_normal((P, n))).astype(np.float64)
+    exp = orc.reduce_fold("MPI_DOUBLE", "MPI_SUM", x)
+    outs = _reduce(gm, orc, "MPI_DOUBLE", "MPI_SUM", x, root=root)
+    assert outs[root].tobytes() == exp.tobytes()
+    outs = _reduce(gm, orc, "MPI_DOUBLE", "MPI_SUM", x, root=root, in_place=True)
+    assert outs[root].tobytes() == exp.tobytes()
+    outs = _reduce(gm, orc, "MPI_DOUBLE", "MPI_SUM", x, in_place=True)
+    for r in range(P):
+        assert outs[r].tobytes() == exp.tobytes()
+
+
+@pytest.mark.parametrize("ncolors", [2, 3])
+def test_allreduce_on_split_communicators(gm, orc, ncolors):
+    rng = np.random.default_rng(ncolors)
+    P, n = 12, 257
+    x = rng.integers(-1000, 1000, (P, n)).astype(np.int32)
+    outs = _reduce(gm, orc, "MPI_INT", "MPI_SUM", x, comm_mode=1, ncolors=ncolors)
+    for c in range(ncolors):
+        members = [r for r in range(P) if r % ncolors == c]
+        exp = orc.reduce_fold("MPI_INT", "MPI_SUM", x[members])
+        for r in members:
+            assert np.array_equal(outs[r], exp)
+    outs = _reduce(gm, orc, "MPI_INT", "MPI_SUM", x, comm_mode=2)
+    exp = orc.reduce_fold("MPI_INT", "MPI_SUM", x)
+    for r in range(P):
+        assert np.array_equal(outs[r], exp)
+
+
+@pytest.mark.parametrize("P,nbytes,root", [(5, 1, 0), (9, 4117, 4), (64, 1 << 20, 63)])
+def test_bcast(gm, orc, P, nbytes, root):
+    stride = (nbytes + 255) // 256 * 256
+    b = gm.DeviceBuffer(stride * P)
+    b.zero()
+    gm.fill_payload(b, stride, stride, 99, 0, P)
+    res = gm.run("coll_bcast", P, [0, nbytes, 0, 0, b, 0, stride, root, 0, 1, 1, 0], timeout_s=60.0)
+    assert res.status == 0, res.describe()
+    exp = orc.payload_bytes(99, root, nbytes)
+    for r in range(P):
+        assert np.array_equal(b.download(np.uint8, nbytes, offset=r * stride), exp), r
+
+
+@pytest.mark.parametrize("P", [4, 16, 128])
+def test_alltoallv_uneven(gm, orc, P):
+    rng = np.random.default_rng(P)
+    counts = rng.integers(0, 3000, (P, P)).astype(np.int32)
+    counts[rng.random((P, P)) < 0.1] = 0
+    srow = counts.sum(axis=1)
+    rcol = counts.sum(axis=0)
+    sstride = (int(srow.max()) * 4 + 255) // 256 * 256 + 256
+    rstride = (int(rcol.max()) * 4 + 255) // 256 * 256 + 256
+    x = [rng.integers(-2 ** 31, 2 ** 31 - 1, int(srow[r]), dtype=np.int32) for r in range(P)]
+    sb = gm.DeviceBuffer(sstride * P)
+    rb = gm.DeviceBuffer(rstride * P)
+    rb.zero()
+    for r in range(P):
+        sb.upload(x[r], offset=r * sstride)
+    cb = gm.DeviceBuffer.from_array(counts)
+    print(f"[gpu] alltoallv P={P}", flush=True)
+    res = gm.run("coll_alltoallv", P, [0, orc.DT["MPI_INT"], cb, 0, sb, rb, sstride, rstride, 0, 0, 1, 0],
+                 timeout_s=60.0)
+    assert res.status == 0, res.describe()
+    sdisp = np.concatenate([np.zeros((P, 1), np.int64), np.cumsum(counts, axis=1)[:, :-1]], axis=1)
+    for r in range(P):
+        exp = np.concatenate([x[s][sdisp[s, r]:sdisp[s, r] + counts[s, r]] for s in range(P)])
+        got = rb.download(np.int32, int(rcol[r]), offset=r * rstride)
+        assert np.array_equal(got, exp), r
+
+
+@pytest.mark.parametrize("kind", range(7))
+def test_block_collectives(gm, kind):
+    P, blk, root = 6, 5, 2
+    cnt = [blk + r for r in range(P)]
+    dsp = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(int)
+    total_v = sum(cnt)
+    sstride = rstride = 4 * 1024
+    sb = gm.DeviceBuffer(sstride * P)
+    rb = gm.DeviceBuffer(rstride * P)
+    rb.zero()
+    vals = {}
+    for r in range(P):
+        v = (np.arange(1024, dtype=np.int32) + 1000 * r)
+        vals[r] = v
+        sb.upload(v, offset=r * sstride)
+    res = gm.run("coll_blocks", P, [0, kind, blk, root, sb, rb, sstride, rstride], timeout_s=60.0)
+    assert res.status == 0, res.describe()
+    got = {r: rb.download(np.int32, 1024, offset=r * rstride) for r in range(P)}
+    if kind == 0:  # gather
+        exp = np.concatenate([vals[r][:blk] for r in range(P)])
+        assert np.array_equal(got[root][:P * blk], exp)
+    elif kind == 1:  # gatherv
+        exp = np.concatenate([vals[r][:blk + r] for r in range(P)])
+        assert np.array_equal(got[root][:total_v], exp)
+    elif kind == 2:  # scatter
+        for r in range(P):
+            assert np.array_equal(got[r][:blk], vals[root][r * blk:(r + 1) * blk])
+    elif kind == 3:  # scatterv
+        for r in range(P):
+            assert np.array_equal(got[r][:blk + r], vals[root][dsp[r]:dsp[r] + cnt[r]])
+    elif kind == 4:  # allgather
+        exp = np.concatenate([vals[r][:blk] for r in range(P)])
+        for r in range(P):
+            assert np.array_equal(got[r][:P * blk], exp)
+    elif kind == 5:  # allgatherv
+        exp = np.concatenate([vals[r][:blk + r] for r in range(P)])
+        for r in range(P):
+            assert np.array_equal(got[r][:total_v], exp)
+    else:  # alltoall
+        for r in range(P):
+            exp = np.concatenate([vals[s][r * blk:(r + 1) * blk] for s in range(P)])
+            assert np.array_equal(got[r][:P * blk], exp)
+
+
+@pytest.mark.parametrize("P,n", [(1, 1000), (8, 100003), (64, 100000000)])
+def test_cpi(gm, orc, P, n):
+    out = gm.DeviceBuffer(8 * (2 + P))
+    out.zero()
+    print(f"[gpu] cpi P={P} n={n}", flush=True)
+    res = gm.run("cpi", P, [0, n, out, 0], timeout_s=120.0)
+    assert res.status == 0, res.describe()
+    got = out.download(np.float64)
+    pi_ref, parts_ref = orc.pi_fold(n, P)
+    assert np.array_equal(got[2:], parts_ref)  # FP64 partial sums bit-exact
+    assert got[0] == pi_ref                   # rank-order fold bit-exact
+    assert abs(got[0] - math.pi) < 2e-12
+    if n == 100000000:
+        gold = json.load(open(os.path.join(GOLDEN, "cpi_mpich.json")))
+        assert abs(got[0] - float(gold["pi_printed"][str(P)])) / math.pi <= 1e-13
