@@ -195,7 +195,7 @@ __device__ inline int coll_begin(MPI_Comm comm, CollCtx* cc) {
 }
 
 __device__ inline void publish(CollCtx& cc, int word, uint64_t v) {
-    if (elected()) cc.tab[cc.me].w[word] = v;
+    cc.tab[cc.me].w[word] = v;
 }
 __device__ inline uint64_t peer_word(CollCtx& cc, int r, int word) { return uniform_u64(cc.tab[r].w[word]); }
 

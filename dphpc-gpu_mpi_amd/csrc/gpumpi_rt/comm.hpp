@@ -33,9 +33,7 @@ __device__ inline void* rank_malloc(uint64_t bytes) {
         return nullptr;
     }
     uint8_t* base = w->arena + (uint64_t)my_rank() * cap;
-    if (elected()) {
-        *top = off + need;
-    }
+    *top = off + need;  // every lane stores the same value (see device.hpp)
     return base + off;
 }
 
@@ -44,7 +42,7 @@ __device__ inline void rank_free(void* p, uint64_t bytes) {
     uint8_t* base = w->arena + (uint64_t)my_rank() * w->cfg.arena_bytes;
     uint64_t* top = &w->arena_top[my_rank()];
     const uint64_t need = (bytes + 255) & ~(uint64_t)255;
-    if ((uint8_t*)p + need == base + *top && elected()) *top = (uint64_t)((uint8_t*)p - base);
+    if ((uint8_t*)p + need == base + *top) *top = (uint64_t)((uint8_t*)p - base);
 }
 
 // ---------------------------------------------------------------------------
@@ -105,11 +103,9 @@ __device__ inline uint8_t* comm_pool_alloc(uint64_t bytes) {
 __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int color, int key) {
     const int psize = parent->size;
     CollSlot* tab = parent->slots;
-    if (elected()) {
-        tab[prank].w[0] = (uint64_t)(uint32_t)color;
-        tab[prank].w[1] = (uint64_t)(uint32_t)key;
-        tab[prank].w[2] = 0;
-    }
+    tab[prank].w[0] = (uint64_t)(uint32_t)color;
+    tab[prank].w[1] = (uint64_t)(uint32_t)key;
+    tab[prank].w[2] = 0;
     comm_barrier(parent, psize);
     // New rank = members of my colour ordered before me by (key, parent rank);
     // leader = the member of my colour with the smallest (key, parent rank).
@@ -154,16 +150,14 @@ __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int co
             if (elected()) ctxid = __hip_atomic_fetch_add(w->next_ctx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             ctxid = uniform_i32(ctxid);
             if (ctxid > 32767) GPUMPI_FAIL(MPI_ERR_COMM, "communicator context ids exhausted");
-            if (elected()) {
-                nc->bar = 0;
-                nc->ctx = ctxid;
-                nc->size = new_size;
-                nc->kind = 2;
-                nc->world_of = world_of;
-                nc->rank_of_world = rank_of_world;
-                nc->slots = slots;
-                tab[prank].w[2] = (uint64_t)(uintptr_t)nc;
-            }
+            nc->bar = 0;
+            nc->ctx = ctxid;
+            nc->size = new_size;
+            nc->kind = 2;
+            nc->world_of = world_of;
+            nc->rank_of_world = rank_of_world;
+            nc->slots = slots;
+            tab[prank].w[2] = (uint64_t)(uintptr_t)nc;
         }
     }
     comm_barrier(parent, psize);
@@ -171,10 +165,8 @@ __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int co
     if (color != MPI_UNDEFINED) {
         CommShared* nc = (CommShared*)(uintptr_t)uniform_u64(tab[leader].w[2]);
         if (nc) {
-            if (elected()) {
-                nc->world_of[new_rank] = my_rank();
-                nc->rank_of_world[my_rank()] = new_rank;
-            }
+            nc->world_of[new_rank] = my_rank();
+            nc->rank_of_world[my_rank()] = new_rank;
             out = (MPI_Comm)nc;
         }
     }

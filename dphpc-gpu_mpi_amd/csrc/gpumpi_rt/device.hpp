@@ -5,8 +5,11 @@
 // executing the same (wave-uniform) code; waves 1..W-1 are helpers that wait
 // in LDS for work posted by wave 0 (helpers.hpp).  Every function in this
 // runtime that is called from the program therefore sees wave-uniform
-// arguments; side effects that must happen once are done by one elected
-// lane and the result is broadcast.
+// arguments.  Plain stores of wave-uniform values are made by every lane
+// (a store made by one lane only may later be read back by the other lanes
+// from their own per-lane view of memory, which the compiler is free to
+// forward from registers); read-modify-write atomics and stores other ranks
+// poll are made by one elected lane and the result broadcast.
 #pragma once
 
 #include <hip/hip_runtime.h>

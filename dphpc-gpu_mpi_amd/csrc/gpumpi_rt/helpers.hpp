@@ -21,9 +21,9 @@ __device__ inline void wg_run(const Cmd& c) {
         return;
     }
     const uint32_t seq = L->cmd_seq + 1;
-    if (elected()) L->cmd = c;
+    L->cmd = c;
     fence_release_wg();
-    if (elected()) __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     run_cmd(c, (int)threadIdx.x, waves * 64);
     const uint32_t target = seq * (uint32_t)(waves - 1);
     int n = 0;
@@ -85,9 +85,9 @@ __device__ inline void helpers_exit() {
     RankLds* L = ctx();
     if (L->waves == 1) return;
     const uint32_t seq = L->cmd_seq + 1;
-    if (elected()) L->cmd.op = CMD_EXIT;
+    L->cmd.op = CMD_EXIT;
     fence_release_wg();
-    if (elected()) __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---------------------------------------------------------------------------

@@ -52,7 +52,7 @@ __device__ inline void rank_body(World* w, int argc, char** argv) {
     rank_init(w);
     if (threadIdx.x < 64) {
         const int ret = Main(argc, argv);
-        if (elected()) w->rank_ret[my_rank()] = ret;
+        w->rank_ret[my_rank()] = ret;
         helpers_exit();
     } else {
         helper_loop();

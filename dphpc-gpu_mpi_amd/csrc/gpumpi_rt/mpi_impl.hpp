@@ -137,10 +137,8 @@ __device__ inline int MPI_Get_count(const MPI_Status* st, MPI_Datatype dt, int* 
 __device__ inline int MPI_Op_create(MPI_User_function* fn, int commute, MPI_Op* op) {
     MPI_Op_impl* o = (MPI_Op_impl*)gpumpi::rt::rank_malloc(sizeof(MPI_Op_impl));
     if (!o) return MPI_ERR_NO_MEM;
-    if (gpumpi::rt::elected()) {
-        o->fn = fn;
-        o->commute = commute;
-    }
+    o->fn = fn;
+    o->commute = commute;
     *op = o;
     return MPI_SUCCESS;
 }
@@ -472,7 +470,7 @@ namespace gpumpi {
 namespace rt {
 __device__ inline MPI_Group group_alloc(int n) {
     MPI_Group_impl* g = (MPI_Group_impl*)rank_malloc(sizeof(int) * (uint64_t)(n + 2));
-    if (g && elected()) g->size = n;
+    if (g) g->size = n;
     return g;
 }
 __device__ inline int group_size(MPI_Group g) {
@@ -522,7 +520,7 @@ __device__ inline int MPI_Group_incl(MPI_Group group, int n, const int ranks[], 
     if (!g) return MPI_ERR_NO_MEM;
     for (int i = 0; i < n; i++) {
         const int v = group_world(group, ranks[i]);
-        if (elected()) g->ranks[i] = v;
+        g->ranks[i] = v;
     }
     *newgroup = g;
     return MPI_SUCCESS;
@@ -544,7 +542,7 @@ __device__ inline int MPI_Group_range_incl(MPI_Group group, int n, int ranges[][
         if ((stride > 0 && first > last) || (stride < 0 && first < last)) continue;
         for (int r = first; stride > 0 ? r <= last : r >= last; r += stride) {
             const int v = group_world(group, r);
-            if (elected()) g->ranks[k] = v;
+            g->ranks[k] = v;
             k++;
         }
     }

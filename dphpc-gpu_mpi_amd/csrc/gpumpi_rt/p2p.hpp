@@ -42,7 +42,7 @@ __device__ __forceinline__ MPI_Request req_handle(int idx) {
     return (MPI_Request)(uintptr_t)(((uint64_t)reqs()[idx].stamp << 16) | (uint64_t)(idx + 1));
 }
 __device__ __forceinline__ int req_index(MPI_Request h) {
-    const uint64_t v = (uint64_t)(uintptr_t)h;
+    const uint64_t v = uniform_u64((uint64_t)(uintptr_t)h);
     const int idx = (int)(v & 0xffff) - 1;
     if (idx < 0 || idx >= ctx()->R) return -1;
     if (reqs()[idx].kind == REQ_FREE || reqs()[idx].stamp != (uint32_t)(v >> 16)) return -1;
@@ -169,15 +169,13 @@ __device__ inline bool try_post_send(int idx) {
     if (eager) {
         copy_part(slot + kSlotHeaderBytes, (const void*)(uintptr_t)q->buf, q->bytes, lane_id(), 64);
     }
-    if (elected()) {
-        hdr->bytes = q->bytes;
-        hdr->flags = eager ? SLOT_EAGER : SLOT_RNDV;
-        hdr->src_world = (uint32_t)me;
-        if (!eager) {
-            hdr->src_addr = q->buf;
-            hdr->done_addr = (uint64_t)(uintptr_t)&world()->done_words[(uint64_t)me * L->R + idx];
-            hdr->done_val = q->stamp;
-        }
+    hdr->bytes = q->bytes;
+    hdr->flags = eager ? SLOT_EAGER : SLOT_RNDV;
+    hdr->src_world = (uint32_t)me;
+    if (!eager) {
+        hdr->src_addr = q->buf;
+        hdr->done_addr = (uint64_t)(uintptr_t)&world()->done_words[(uint64_t)me * L->R + idx];
+        hdr->done_val = q->stamp;
     }
     // Envelope last: the release fence orders the payload and header (every
     // lane's stores) before it.
