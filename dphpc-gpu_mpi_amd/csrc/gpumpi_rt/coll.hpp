@@ -163,6 +163,11 @@ __device__ inline void run_cmd(const Cmd& c, int t, int nt) {
     case CMD_ALLTOALLV:
         run_segments(c, t, nt);
         break;
+    case CMD_USER: {
+        auto fn = (void (*)(const void*, int, int))(uintptr_t)c.a[0];
+        fn(&c.a[1], t, nt);
+        break;
+    }
     default:
         break;
     }

@@ -160,7 +160,7 @@ struct RankLds {
     uint64_t bounce_top; // staging bump offset for private rendezvous sends
     int bounce_users;
     // offsets of the variable arrays from the LDS base
-    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits;
+    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user;
     uint32_t lds_bytes;
 };
 
@@ -181,9 +181,9 @@ __device__ __forceinline__ uint64_t* free_bits() {
 // Byte size of the LDS image for a configuration (host and device agree).
 __host__ __device__ inline uint32_t align_up(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
 struct LdsLayout {
-    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, total;
+    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user, total;
 };
-__host__ __device__ inline LdsLayout lds_layout(int nranks, int R) {
+__host__ __device__ inline LdsLayout lds_layout(int nranks, int R, int user_bytes) {
     LdsLayout L;
     uint32_t o = align_up((uint32_t)sizeof(RankLds), 16);
     L.off_reqs = o; o = align_up(o + (uint32_t)(sizeof(Req) * R), 16);
@@ -192,6 +192,7 @@ __host__ __device__ inline LdsLayout lds_layout(int nranks, int R) {
     L.off_tx = o; o = align_up(o + (uint32_t)(sizeof(TxChan) * nranks), 16);
     L.off_unmatched = o; o = align_up(o + (uint32_t)(2 * nranks), 16);
     L.off_freebits = o; o = align_up(o + (uint32_t)(8 * ((R + 63) / 64)), 16);
+    L.off_user = o; o = align_up(o + (uint32_t)user_bytes, 16);
     L.total = o;
     return L;
 }

@@ -18,7 +18,7 @@ __device__ __forceinline__ int rank_of_block(const Config& cfg, int b) {
 
 __device__ inline void rank_init(World* w) {
     const Config& cfg = w->cfg;
-    const LdsLayout lay = lds_layout(cfg.nranks, cfg.max_requests);
+    const LdsLayout lay = lds_layout(cfg.nranks, cfg.max_requests, cfg.user_lds_bytes);
     // Zero the whole LDS image with every thread.
     uint32_t* z = (uint32_t*)gpumpi_lds_base;
     for (uint32_t i = threadIdx.x; i < lay.total / 4; i += blockDim.x) z[i] = 0;
@@ -42,6 +42,7 @@ __device__ inline void rank_init(World* w) {
         L->off_tx = lay.off_tx;
         L->off_unmatched = lay.off_unmatched;
         L->off_freebits = lay.off_freebits;
+        L->off_user = lay.off_user;
         L->lds_bytes = lay.total;
     }
     __syncthreads();
@@ -89,7 +90,35 @@ __device__ inline T* gpumpi_user_args() { return (T*)gpumpi::rt::world()->user_a
 // This rank's world rank and the wall clock, without MPI calls.
 __device__ inline int gpumpi_world_rank() { return gpumpi::rt::my_rank(); }
 __device__ inline uint64_t gpumpi_ticks() { return gpumpi::rt::uniform_u64(gpumpi::rt::wall_ticks()); }
-// Run `f(t, nt)` on every thread of the rank (all waves), e.g. for local
-// data-parallel compute of a rank program.  `f` must be a device lambda
-// without captures of private (stack) memory.
 __device__ inline bool gpumpi_is_leader_lane() { return gpumpi::rt::elected(); }
+
+// LDS the launch reserved for the rank program (user_lds_bytes of the
+// launch configuration); shared by every wave of the rank.
+template <typename T>
+__device__ inline T* gpumpi_user_lds() {
+    return (T*)(gpumpi::rt::gpumpi_lds_base + gpumpi::rt::ctx()->off_user);
+}
+
+namespace gpumpi {
+namespace rt {
+template <typename F>
+__device__ void user_trampoline(const void* storage, int t, int nt) {
+    (*(const F*)storage)(t, nt);
+}
+}  // namespace rt
+}  // namespace gpumpi
+
+// Run `f(t, nt)` on every thread t of the rank's workgroup (nt threads,
+// every wave), e.g. the local compute phases of a rank program; returns
+// when all threads are done.  `f` is copied by value (at most 72 bytes of
+// captures); captured pointers must be global (or user LDS) memory.
+template <typename F>
+__device__ inline void gpumpi_parallel(const F& f) {
+    static_assert(sizeof(F) <= 9 * sizeof(uint64_t), "gpumpi_parallel: captures exceed 72 bytes");
+    gpumpi::rt::Cmd c{};
+    c.op = gpumpi::rt::CMD_USER;
+    c.a[0] = (uint64_t)(uintptr_t)&gpumpi::rt::user_trampoline<F>;
+    __builtin_memcpy(&c.a[1], &f, sizeof(F));
+    gpumpi::rt::wg_run(c);
+}
+// Barrier of the rank's own waves is implicit at the end of gpumpi_parallel.

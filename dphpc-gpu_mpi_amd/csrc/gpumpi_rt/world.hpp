@@ -40,6 +40,7 @@ struct Config {
     uint64_t comm_pool_bytes;
     int lds_bytes;        // dynamic LDS per workgroup
     int spin_abort_period; // polls between abort-word checks
+    int user_lds_bytes;    // LDS reserved for the rank program (gpumpi_user_lds)
 };
 
 // Envelope word of a slot: [63:32] sequence+1, [31:16] context, [15:0] tag.

@@ -238,7 +238,8 @@ int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
     g.wall_hz = (uint64_t)wall_khz * 1000ull;
     if (g.wall_hz == 0) g.wall_hz = 100000000ull;
     g.num_xcd = (cus % 8 == 0 && cus >= 64) ? 8 : 1;
-    const LdsLayout lay = lds_layout(g.nranks, g.max_requests);
+    g.user_lds_bytes = c.user_lds_bytes;
+    const LdsLayout lay = lds_layout(g.nranks, g.max_requests, g.user_lds_bytes);
     g.lds_bytes = (int)lay.total;
     *lds_bytes = (int)lay.total;
     int max_lds = 0;

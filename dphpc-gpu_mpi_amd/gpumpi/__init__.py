@@ -50,7 +50,7 @@ class Config(ctypes.Structure):
                 ("stack_size", ctypes.c_size_t), ("heap_bytes", ctypes.c_size_t),
                 ("staging_bytes", ctypes.c_size_t), ("comm_pool_bytes", ctypes.c_size_t),
                 ("timeout_s", ctypes.c_double), ("verbose", ctypes.c_int),
-                ("reserved", ctypes.c_int * 7)]
+                ("user_lds_bytes", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
 
 
 class Result(ctypes.Structure):

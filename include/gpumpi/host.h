@@ -42,7 +42,8 @@ typedef struct gpumpi_config {
     size_t comm_pool_bytes;
     double timeout_s;
     int verbose;
-    int reserved[7];
+    int user_lds_bytes;   /* LDS per rank for the program (gpumpi_user_lds) */
+    int reserved[6];
 } gpumpi_config;
 
 /* What a launch reports back. */
