@@ -80,6 +80,13 @@ def build(jobs=8, force=False, verbose=True):
     if force or _stale(progs, prog_objs + [core]):
         subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", progs] + prog_objs +
                        ["-L", LIB, "-lgpumpi", "-Wl,-rpath,$ORIGIN"], check=True)
+    # The converted-program example: a standalone executable with the
+    # reference's `---gpumpi` command line.
+    exe = os.path.join(LIB, "cpi")
+    ex_src = os.path.join(PKG, "examples", "cpi_main.cpp")
+    if force or _stale(exe, [ex_src, core] + _headers()):
+        subprocess.run([HIPCC] + COMMON + ["-x", "hip", ex_src, "-o", exe, "-L", LIB, "-lgpumpi",
+                                           "-Wl,-rpath,$ORIGIN"], check=True, capture_output=True)
     return core, progs
 
 
