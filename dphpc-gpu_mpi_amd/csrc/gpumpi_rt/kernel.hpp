@@ -46,6 +46,49 @@ __device__ inline void rank_init(World* w) {
         L->lds_bytes = lay.total;
     }
     __syncthreads();
+    // Resume the channel cursors of the previous launch (see World::saves).
+    const int rank = ctx()->rank;
+    if (w->restore) {
+        const uint8_t* sv = w->saves + (uint64_t)rank * w->save_bytes;
+        const uint32_t* src = (const uint32_t*)(sv + 64);
+        uint32_t* rx = (uint32_t*)rxc();
+        const uint32_t nrx = (uint32_t)cfg.nranks * 4, ntx = (uint32_t)cfg.nranks * 2;
+        for (uint32_t i = threadIdx.x; i < nrx; i += blockDim.x) rx[i] = src[i];
+        uint32_t* tx = (uint32_t*)txc();
+        for (uint32_t i = threadIdx.x; i < ntx; i += blockDim.x) tx[i] = src[nrx + i];
+        if (threadIdx.x == 0) ctx()->stamp = *(const uint32_t*)sv;
+    }
+    if (threadIdx.x == 0) {
+        w->arena_top[rank] = 0;
+        if (rank == 0) {  // ordered before any use by MPI_Init's barrier
+            __hip_atomic_store(w->comm_pool_top, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(w->next_ctx, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+}
+
+// Wave 0 saves the channel cursors and the stamp at the rank's exit.  A
+// message that arrived but was never received would be matched by the next
+// launch, so such a rank flags the world for a reset instead.
+__device__ inline void rank_save(World* w) {
+    bool stale = false;
+    for (int src = lane_id(); src < ctx()->nranks; src += 64) {
+        const uint32_t h = rxc()[src].head;
+        const uint64_t e = ld_relaxed_agent(&((const SlotHdr*)slot_at(my_rank(), src, h))->env);
+        if (env_seq(e) == h + 1) stale = true;
+    }
+    if (ballot(stale) && elected())
+        __hip_atomic_store(w->abort_word + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const Config& cfg = w->cfg;
+    uint8_t* sv = w->saves + (uint64_t)my_rank() * w->save_bytes;
+    uint32_t* dst = (uint32_t*)(sv + 64);
+    const uint32_t* rx = (const uint32_t*)rxc();
+    const uint32_t* tx = (const uint32_t*)txc();
+    const uint32_t nrx = (uint32_t)cfg.nranks * 4, ntx = (uint32_t)cfg.nranks * 2;
+    for (uint32_t i = (uint32_t)lane_id(); i < nrx; i += 64) dst[i] = rx[i];
+    for (uint32_t i = (uint32_t)lane_id(); i < ntx; i += 64) dst[nrx + i] = tx[i];
+    *(uint32_t*)sv = ctx()->stamp;
 }
 
 template <int (*Main)(int, char**)>
@@ -54,6 +97,7 @@ __device__ inline void rank_body(World* w, int argc, char** argv) {
     if (threadIdx.x < 64) {
         const int ret = Main(argc, argv);
         w->rank_ret[my_rank()] = ret;
+        rank_save(w);
         helpers_exit();
     } else {
         helper_loop();

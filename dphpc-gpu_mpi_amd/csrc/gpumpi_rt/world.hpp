@@ -114,7 +114,19 @@ struct World {
     int* abort_word;       // pinned host
     ErrorRecord* err;      // pinned host
     void* user_args;       // program arguments (opaque)
+    // Channel cursors and request stamps survive between launches of the same
+    // world shape, so a launch needs no reset of the P*P*Q slots: each rank
+    // restores them at start and saves them at exit.
+    uint8_t* saves;        // [P * save_bytes]
+    uint64_t save_bytes;
+    int restore;           // 0: first launch after a reset (start from zero)
+    int pad_w;
 };
+
+// Per-rank save block: stamp, then RxChan[P] and TxChan[P].
+__host__ __device__ inline uint64_t save_block_bytes(int nranks) {
+    return 64 + (uint64_t)nranks * (16 + 8);
+}
 
 }  // namespace rt
 }  // namespace gpumpi

@@ -67,6 +67,9 @@ struct WorldAlloc {
     ErrorRecord* err = nullptr; // pinned host
     World* dworld = nullptr;
     uint8_t* world_tables = nullptr;  // world_of, rank_of_world, slots of WORLD; self tables
+    uint8_t* saves = nullptr;         // per-rank channel cursors between launches
+    uint64_t save_bytes = 0;
+    bool need_reset = true;           // next launch starts the channels from zero
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -74,7 +77,7 @@ struct WorldAlloc {
         auto f = [](void* p) { if (p) (void)hipFree(p); };
         f(slots); f(chan_head); f(done_words); f(staging); f(arena); f(arena_top);
         f(comm_world); f(comm_self); f(comm_pool); f(comm_pool_top); f(next_ctx); f(rank_ret);
-        f(rank_stats); f(dworld); f(world_tables);
+        f(rank_stats); f(dworld); f(world_tables); f(saves);
         if (abort_word) (void)hipHostFree(abort_word);
         if (err) (void)hipHostFree(err);
         *this = WorldAlloc();
@@ -116,6 +119,9 @@ int alloc_world(const Config& cfg) {
     // SELF tables: slots[P] (one per rank).
     const uint64_t tables = 2 * P * sizeof(int) + 256 + 2 * P * sizeof(CollSlot);
     HIPOK(hipMalloc(&W.world_tables, tables));
+    W.save_bytes = save_block_bytes(cfg.nranks);
+    HIPOK(hipMalloc(&W.saves, P * W.save_bytes));
+    W.need_reset = true;
     HIPOK(hipHostMalloc((void**)&W.abort_word, 64, hipHostMallocCoherent));
     HIPOK(hipHostMalloc((void**)&W.err, sizeof(ErrorRecord), hipHostMallocCoherent));
     HIPOK(hipStreamCreateWithFlags(&W.stream, hipStreamNonBlocking));
@@ -132,6 +138,9 @@ int init_world(const Config& cfg, void* user_args) {
     WorldAlloc& W = g_world;
     const uint64_t P = (uint64_t)cfg.nranks;
     hipStream_t s = W.stream;
+    const bool reset = W.need_reset;
+    W.need_reset = true;  // cleared again only when this launch ends cleanly
+    if (reset) {
     HIPOK(hipMemsetAsync(W.slots, 0, W.slots_bytes, s));
     HIPOK(hipMemsetAsync(W.chan_head, 0, P * P * sizeof(uint32_t), s));
     HIPOK(hipMemsetAsync(W.done_words, 0, P * (uint64_t)cfg.max_requests * sizeof(uint32_t), s));
@@ -168,8 +177,11 @@ int init_world(const Config& cfg, void* user_args) {
         cs[r].slots = sslots + r;
     }
     HIPOK(hipMemcpyAsync(W.comm_self, cs.data(), P * sizeof(CommShared), hipMemcpyHostToDevice, s));
+    HIPOK(hipMemsetAsync(W.saves, 0, P * W.save_bytes, s));
+    }  // reset
 
-    *W.abort_word = 0;
+    W.abort_word[0] = 0;
+    W.abort_word[1] = 0;  // set by a rank that exits with undelivered messages
     std::memset((void*)W.err, 0, sizeof(ErrorRecord));
 
     World hw{};
@@ -190,6 +202,9 @@ int init_world(const Config& cfg, void* user_args) {
     hw.abort_word = W.abort_word;
     hw.err = W.err;
     hw.user_args = user_args;
+    hw.saves = W.saves;
+    hw.save_bytes = W.save_bytes;
+    hw.restore = reset ? 0 : 1;
     HIPOK(hipMemcpyAsync(W.dworld, &hw, sizeof hw, hipMemcpyHostToDevice, s));
     HIPOK(hipStreamSynchronize(s));
     return GPUMPI_OK;
@@ -429,6 +444,10 @@ int gpumpi_run(const gpumpi_config* cfg_in, const void* kernel, int argc, char**
     R.mpi_error_count = e.count;
     std::memcpy(R.mpi_error_msg, e.msg, sizeof R.mpi_error_msg);
     R.mpi_error_msg[sizeof R.mpi_error_msg - 1] = 0;
+    // Channel cursors are reusable only after a clean launch that left no
+    // undelivered message behind.
+    if (!R.timed_out && !R.mpi_error && __atomic_load_n(W.abort_word + 1, __ATOMIC_SEQ_CST) == 0)
+        W.need_reset = false;
     if (R.timed_out) {
         set_err("watchdog: launch exceeded %.1f s and was aborted", cfg_in->timeout_s);
         return R.status = GPUMPI_ETIMEOUT;
