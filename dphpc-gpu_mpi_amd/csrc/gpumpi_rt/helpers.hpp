@@ -36,6 +36,29 @@ __device__ inline void wg_run(const Cmd& c) {
     }
 }
 
+// A device functor run by every thread of the rank (CMD_USER): the functor
+// travels by value inside the LDS command (at most 72 bytes).
+template <typename F>
+__device__ void user_trampoline(const void* storage, int t, int nt) {
+    (*(const F*)storage)(t, nt);
+}
+template <typename F>
+__device__ inline void par_all_waves(const F& f) {
+    static_assert(sizeof(F) <= 9 * sizeof(uint64_t), "parallel body captures exceed 72 bytes");
+    Cmd c{};
+    c.op = CMD_USER;
+    c.a[0] = (uint64_t)(uintptr_t)&user_trampoline<F>;
+    __builtin_memcpy(&c.a[1], &f, sizeof(F));
+    wg_run(c);
+}
+// Same, but on wave 0 alone when the work touches fewer than
+// helper_min_bytes of memory.
+template <typename F>
+__device__ inline void par_run(const F& f, uint64_t bytes) {
+    if (bytes < (uint64_t)ctx()->helper_min_bytes || ctx()->waves == 1) f(lane_id(), 64);
+    else par_all_waves(f);
+}
+
 // Copy by this rank: wave 0 alone for small or private-destination copies,
 // every wave for large ones.
 __device__ inline void rank_copy(void* dst, const void* src, uint64_t n) {

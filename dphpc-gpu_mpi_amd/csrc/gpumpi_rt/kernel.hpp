@@ -143,26 +143,12 @@ __device__ inline T* gpumpi_user_lds() {
     return (T*)(gpumpi::rt::gpumpi_lds_base + gpumpi::rt::ctx()->off_user);
 }
 
-namespace gpumpi {
-namespace rt {
-template <typename F>
-__device__ void user_trampoline(const void* storage, int t, int nt) {
-    (*(const F*)storage)(t, nt);
-}
-}  // namespace rt
-}  // namespace gpumpi
-
 // Run `f(t, nt)` on every thread t of the rank's workgroup (nt threads,
 // every wave), e.g. the local compute phases of a rank program; returns
 // when all threads are done.  `f` is copied by value (at most 72 bytes of
 // captures); captured pointers must be global (or user LDS) memory.
 template <typename F>
 __device__ inline void gpumpi_parallel(const F& f) {
-    static_assert(sizeof(F) <= 9 * sizeof(uint64_t), "gpumpi_parallel: captures exceed 72 bytes");
-    gpumpi::rt::Cmd c{};
-    c.op = gpumpi::rt::CMD_USER;
-    c.a[0] = (uint64_t)(uintptr_t)&gpumpi::rt::user_trampoline<F>;
-    __builtin_memcpy(&c.a[1], &f, sizeof(F));
-    gpumpi::rt::wg_run(c);
+    gpumpi::rt::par_all_waves(f);
 }
 // Barrier of the rank's own waves is implicit at the end of gpumpi_parallel.

@@ -16,7 +16,8 @@ __device__ inline MPI_Comm pick_comm(int mode, int ncolors) {
 
 // args: [1] dtype, [2] op index (0..11), [3] count, [4] in base, [5] out
 // base, [6] stride, [7] root (-1: Allreduce), [8] comm mode, [9] ncolors,
-// [10] iters, [11] times (double[P]), [12] in place
+// [10] iters, [11] times (double[P]), [12] in place, [13] wire format
+// (0 exact, 1 BF16, 2 FP8 e4m3: compressed Allreduce, MPI_FLOAT/SUM only)
 GPUMPI_PROGRAM(coll_reduce) {
     MPI_Init(&argc, &argv);
     int wrank;
@@ -31,6 +32,7 @@ GPUMPI_PROGRAM(coll_reduce) {
     const int iters = (int)prog_int(10);
     double* times = prog_ptr<double>(11);
     const bool in_place = prog_int(12) != 0;
+    const int wire = (int)prog_int(13);
     MPI_Comm comm = pick_comm((int)prog_int(8), (int)prog_int(9));
     int crank;
     MPI_Comm_rank(comm, &crank);
@@ -38,7 +40,8 @@ GPUMPI_PROGRAM(coll_reduce) {
     const double t0 = MPI_Wtime();
     for (int it = 0; it < iters; it++) {
         const void* sb = in_place && (root < 0 || crank == root) ? MPI_IN_PLACE : in;
-        if (root < 0) MPI_Allreduce(sb, out, count, dt, op, comm);
+        if (root < 0 && wire) GPUMPI_Allreduce_wire(sb, out, count, dt, op, comm, wire);
+        else if (root < 0) MPI_Allreduce(sb, out, count, dt, op, comm);
         else MPI_Reduce(sb, out, count, dt, op, root, comm);
     }
     const double t1 = MPI_Wtime();
