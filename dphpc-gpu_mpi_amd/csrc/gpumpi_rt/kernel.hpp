@@ -48,8 +48,8 @@ __device__ inline void rank_init(World* w) {
     __syncthreads();
     // Resume the channel cursors of the previous launch (see World::saves).
     const int rank = ctx()->rank;
-    if (w->restore) {
-        const uint8_t* sv = w->saves + (uint64_t)rank * w->save_bytes;
+    const uint8_t* sv = w->saves + (uint64_t)rank * w->save_bytes;
+    if (((const uint32_t*)sv)[1] == kSaveMagic) {
         const uint32_t* src = (const uint32_t*)(sv + 64);
         uint32_t* rx = (uint32_t*)rxc();
         const uint32_t nrx = (uint32_t)cfg.nranks * 4, ntx = (uint32_t)cfg.nranks * 2;
@@ -89,6 +89,7 @@ __device__ inline void rank_save(World* w) {
     for (uint32_t i = (uint32_t)lane_id(); i < nrx; i += 64) dst[i] = rx[i];
     for (uint32_t i = (uint32_t)lane_id(); i < ntx; i += 64) dst[nrx + i] = tx[i];
     *(uint32_t*)sv = ctx()->stamp;
+    ((uint32_t*)sv)[1] = kSaveMagic;
 }
 
 template <int (*Main)(int, char**)>

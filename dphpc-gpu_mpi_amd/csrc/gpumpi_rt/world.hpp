@@ -119,11 +119,11 @@ struct World {
     // restores them at start and saves them at exit.
     uint8_t* saves;        // [P * save_bytes]
     uint64_t save_bytes;
-    int restore;           // 0: first launch after a reset (start from zero)
-    int pad_w;
 };
 
-// Per-rank save block: stamp, then RxChan[P] and TxChan[P].
+// Per-rank save block: stamp, validity magic, then RxChan[P] and TxChan[P]
+// from byte 64.  A zeroed block (after a host reset) means "start fresh".
+constexpr uint32_t kSaveMagic = 0x53415645u;
 __host__ __device__ inline uint64_t save_block_bytes(int nranks) {
     return 64 + (uint64_t)nranks * (16 + 8);
 }

@@ -204,7 +204,6 @@ int init_world(const Config& cfg, void* user_args) {
     hw.user_args = user_args;
     hw.saves = W.saves;
     hw.save_bytes = W.save_bytes;
-    hw.restore = reset ? 0 : 1;
     HIPOK(hipMemcpyAsync(W.dworld, &hw, sizeof hw, hipMemcpyHostToDevice, s));
     HIPOK(hipStreamSynchronize(s));
     return GPUMPI_OK;
@@ -356,62 +355,13 @@ int gpumpi_max_ranks(const gpumpi_config* cfg, const void* kernel) {
     return per_cu * cus;
 }
 
-int gpumpi_run(const gpumpi_config* cfg_in, const void* kernel, int argc, char** argv, void* user_args,
-               gpumpi_result* res) {
-    gpumpi_result local{};
-    gpumpi_result& R = res ? *res : local;
-    std::memset(&R, 0, sizeof R);
-    Config cfg;
-    int lds = 0;
-    int rc = build_config(cfg_in, &cfg, &lds);
-    if (rc) return R.status = rc;
-    R.lds_bytes = lds;
-    if (lds > 48 * 1024) HIPOK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-    const int maxr = gpumpi_max_ranks(cfg_in, kernel);
-    if (maxr < cfg.nranks) {
-        set_err("%d ranks of %d threads cannot all be resident (max %d): grid barriers would deadlock",
-                cfg.nranks, cfg.threads, maxr);
-        return R.status = GPUMPI_ETOOBIG;
-    }
-    if ((rc = alloc_world(cfg))) return R.status = rc;
-    if ((rc = init_world(cfg, user_args))) return R.status = rc;
-    if (cfg_in->stack_size) HIPOK(hipDeviceSetLimit(hipLimitStackSize, cfg_in->stack_size));
+}  // extern "C" (reopened below)
 
-    // argv in device memory (main.cu:23-42 used managed memory).
-    size_t strbytes = 0;
-    for (int i = 0; i < argc; i++) strbytes += std::strlen(argv[i]) + 1;
-    const size_t ptrbytes = (size_t)(argc + 1) * sizeof(char*);
-    std::vector<uint8_t> img(ptrbytes + strbytes + 1);
-    char* dargv = nullptr;
-    HIPOK(hipMalloc(&dargv, img.size()));
-    {
-        char** ptrs = (char**)img.data();
-        size_t off = ptrbytes;
-        for (int i = 0; i < argc; i++) {
-            ptrs[i] = dargv + off;
-            std::memcpy(img.data() + off, argv[i], std::strlen(argv[i]) + 1);
-            off += std::strlen(argv[i]) + 1;
-        }
-        ptrs[argc] = nullptr;
-    }
+namespace {
+
+// Waits for ev1 with the watchdog; returns a GPUMPI status.
+int wait_launch(const gpumpi_config* cfg_in, gpumpi_result& R) {
     WorldAlloc& W = g_world;
-    HIPOK(hipMemcpyAsync(dargv, img.data(), img.size(), hipMemcpyHostToDevice, W.stream));
-
-    World* wptr = W.dworld;
-    char** argvp = (char**)dargv;
-    void* params[] = {(void*)&wptr, (void*)&argc, (void*)&argvp};
-    HIPOK(hipEventRecord(W.ev0, W.stream));
-    hipError_t le = hipLaunchCooperativeKernel(kernel, dim3((unsigned)cfg.nranks), dim3((unsigned)cfg.threads),
-                                               params, (unsigned)lds, W.stream);
-    if (le != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(dargv);
-        set_err("cooperative launch of %d x %d failed: %s", cfg.nranks, cfg.threads, hipGetErrorString(le));
-        return R.status = (le == hipErrorCooperativeLaunchTooLarge) ? GPUMPI_ETOOBIG : GPUMPI_EHIP;
-    }
-    HIPOK(hipEventRecord(W.ev1, W.stream));
-
-    // Host polling loop with watchdog (main.cu:293-301 polled for libc work).
     const auto t0 = std::chrono::steady_clock::now();
     bool fired = false;
     for (;;) {
@@ -419,7 +369,7 @@ int gpumpi_run(const gpumpi_config* cfg_in, const void* kernel, int argc, char**
         if (q == hipSuccess) break;
         if (q != hipErrorNotReady) {
             set_err("kernel failed: %s", hipGetErrorString(q));
-            return R.status = GPUMPI_EHIP;
+            return GPUMPI_EHIP;
         }
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (!fired && cfg_in->timeout_s > 0 && el > cfg_in->timeout_s) {
@@ -432,9 +382,14 @@ int gpumpi_run(const gpumpi_config* cfg_in, const void* kernel, int argc, char**
     float ms = 0.f;
     HIPOK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
     R.kernel_ms = ms;
+    return GPUMPI_OK;
+}
+
+// Collects return codes and the device error record after a launch.
+int finish_launch(const gpumpi_config* cfg_in, const Config& cfg, gpumpi_result& R) {
+    WorldAlloc& W = g_world;
     std::vector<int> rets(cfg.nranks);
     HIPOK(hipMemcpy(rets.data(), W.rank_ret, sizeof(int) * cfg.nranks, hipMemcpyDeviceToHost));
-    HIPOK(hipFree(dargv));
     for (int r : rets)
         if (r != 0) R.any_rank_failed = 1;
     const ErrorRecord& e = *W.err;
@@ -448,15 +403,144 @@ int gpumpi_run(const gpumpi_config* cfg_in, const void* kernel, int argc, char**
     // undelivered message behind.
     if (!R.timed_out && !R.mpi_error && __atomic_load_n(W.abort_word + 1, __ATOMIC_SEQ_CST) == 0)
         W.need_reset = false;
+    else
+        W.need_reset = true;
     if (R.timed_out) {
         set_err("watchdog: launch exceeded %.1f s and was aborted", cfg_in->timeout_s);
-        return R.status = GPUMPI_ETIMEOUT;
+        return GPUMPI_ETIMEOUT;
     }
     if (R.mpi_error) {
         set_err("device MPI error %d on rank %d: %s", e.code, e.rank, e.msg);
-        return R.status = GPUMPI_EDEVICE;
+        return GPUMPI_EDEVICE;
     }
-    return R.status = GPUMPI_OK;
+    return GPUMPI_OK;
+}
+
+// Everything a launch needs: checked configuration, world, device argv.
+struct Prepared {
+    Config cfg;
+    int lds = 0;
+    char* dargv = nullptr;
+    World* wptr = nullptr;
+    char** argvp = nullptr;
+    int argc = 0;
+    ~Prepared() {
+        if (dargv) (void)hipFree(dargv);
+    }
+};
+
+int prepare_launch(const gpumpi_config* cfg_in, const void* kernel, int argc, char** argv, void* user_args,
+                   gpumpi_result& R, Prepared& L) {
+    int rc = build_config(cfg_in, &L.cfg, &L.lds);
+    if (rc) return rc;
+    R.lds_bytes = L.lds;
+    if (L.lds > 48 * 1024) HIPOK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, L.lds));
+    const int maxr = gpumpi_max_ranks(cfg_in, kernel);
+    if (maxr < L.cfg.nranks) {
+        set_err("%d ranks of %d threads cannot all be resident (max %d): grid barriers would deadlock",
+                L.cfg.nranks, L.cfg.threads, maxr);
+        return GPUMPI_ETOOBIG;
+    }
+    if ((rc = alloc_world(L.cfg))) return rc;
+    if ((rc = init_world(L.cfg, user_args))) return rc;
+    if (cfg_in->stack_size) HIPOK(hipDeviceSetLimit(hipLimitStackSize, cfg_in->stack_size));
+    // argv in device memory (main.cu:23-42 used managed memory).
+    size_t strbytes = 0;
+    for (int i = 0; i < argc; i++) strbytes += std::strlen(argv[i]) + 1;
+    const size_t ptrbytes = (size_t)(argc + 1) * sizeof(char*);
+    std::vector<uint8_t> img(ptrbytes + strbytes + 1);
+    HIPOK(hipMalloc(&L.dargv, img.size()));
+    char** ptrs = (char**)img.data();
+    size_t off = ptrbytes;
+    for (int i = 0; i < argc; i++) {
+        ptrs[i] = L.dargv + off;
+        std::memcpy(img.data() + off, argv[i], std::strlen(argv[i]) + 1);
+        off += std::strlen(argv[i]) + 1;
+    }
+    ptrs[argc] = nullptr;
+    HIPOK(hipMemcpy(L.dargv, img.data(), img.size(), hipMemcpyHostToDevice));
+    L.wptr = g_world.dworld;
+    L.argvp = (char**)L.dargv;
+    L.argc = argc;
+    return GPUMPI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpumpi_run(const gpumpi_config* cfg_in, const void* kernel, int argc, char** argv, void* user_args,
+               gpumpi_result* res) {
+    gpumpi_result local{};
+    gpumpi_result& R = res ? *res : local;
+    std::memset(&R, 0, sizeof R);
+    Prepared L;
+    int rc = prepare_launch(cfg_in, kernel, argc, argv, user_args, R, L);
+    if (rc) return R.status = rc;
+    WorldAlloc& W = g_world;
+    void* params[] = {(void*)&L.wptr, (void*)&L.argc, (void*)&L.argvp};
+    HIPOK(hipEventRecord(W.ev0, W.stream));
+    hipError_t le = hipLaunchCooperativeKernel(kernel, dim3((unsigned)L.cfg.nranks), dim3((unsigned)L.cfg.threads),
+                                               params, (unsigned)L.lds, W.stream);
+    if (le != hipSuccess) {
+        (void)hipGetLastError();
+        set_err("cooperative launch of %d x %d failed: %s", L.cfg.nranks, L.cfg.threads, hipGetErrorString(le));
+        return R.status = (le == hipErrorCooperativeLaunchTooLarge) ? GPUMPI_ETOOBIG : GPUMPI_EHIP;
+    }
+    HIPOK(hipEventRecord(W.ev1, W.stream));
+    if ((rc = wait_launch(cfg_in, R))) return R.status = rc;
+    return R.status = finish_launch(cfg_in, L.cfg, R);
+}
+
+int gpumpi_run_graph(const gpumpi_config* cfg_in, const void* kernel, int argc, char** argv, void* user_args,
+                     int replays, gpumpi_result* res, double* replay_ms) {
+    gpumpi_result local{};
+    gpumpi_result& R = res ? *res : local;
+    std::memset(&R, 0, sizeof R);
+    if (replays < 1) {
+        set_err("replays must be >= 1");
+        return R.status = GPUMPI_EINVAL;
+    }
+    Prepared L;
+    int rc = prepare_launch(cfg_in, kernel, argc, argv, user_args, R, L);
+    if (rc) return R.status = rc;
+    WorldAlloc& W = g_world;
+    // Capture one launch of the rank kernel (residency was checked above, as
+    // the cooperative launch would) and replay it.
+    void* params[] = {(void*)&L.wptr, (void*)&L.argc, (void*)&L.argvp};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPOK(hipStreamBeginCapture(W.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipLaunchKernel(kernel, dim3((unsigned)L.cfg.nranks), dim3((unsigned)L.cfg.threads), params,
+                                    (size_t)L.lds, W.stream);
+    hipError_t ce = hipStreamEndCapture(W.stream, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+        set_err("graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
+        if (graph) (void)hipGraphDestroy(graph);
+        return R.status = GPUMPI_EHIP;
+    }
+    HIPOK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    double total = 0;
+    for (int i = 0; i < replays && rc == GPUMPI_OK; i++) {
+        W.abort_word[0] = 0;
+        W.abort_word[1] = 0;
+        std::memset((void*)W.err, 0, sizeof(ErrorRecord));
+        HIPOK(hipEventRecord(W.ev0, W.stream));
+        HIPOK(hipGraphLaunch(exec, W.stream));
+        HIPOK(hipEventRecord(W.ev1, W.stream));
+        if ((rc = wait_launch(cfg_in, R))) break;
+        rc = finish_launch(cfg_in, L.cfg, R);
+        if (replay_ms) replay_ms[i] = R.kernel_ms;
+        total += R.kernel_ms;
+        if (W.need_reset && i + 1 < replays && rc == GPUMPI_OK) {
+            set_err("a replay left undelivered messages; later replays would see them");
+            rc = GPUMPI_EDEVICE;
+        }
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    R.kernel_ms = total / replays;
+    return R.status = rc;
 }
 
 int gpumpi_main(int argc, char** argv, const void* kernel) {

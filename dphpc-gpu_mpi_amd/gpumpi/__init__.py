@@ -33,11 +33,11 @@ OPS = ["MPI_MAX", "MPI_MIN", "MPI_SUM", "MPI_PROD", "MPI_LAND", "MPI_BAND", "MPI
 OP = {n: i for i, n in enumerate(OPS)}
 
 # Exported symbols of each library (the C-ABI surface).
-CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpumpi_main",
+CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpumpi_run_graph", "gpumpi_main",
                 "gpumpi_device_alloc", "gpumpi_device_free", "gpumpi_memcpy_htod",
                 "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error"]
 PROGS_SYMBOLS = ["gpumpi_prog_lookup", "gpumpi_prog_count", "gpumpi_prog_name",
-                 "gpumpi_fill_payload", "gpumpi_verify_payload"]
+                 "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal"]
 
 STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ETOOBIG", 4: "EDEVICE", 5: "ETIMEOUT", 6: "ENOMEM"}
 
@@ -92,6 +92,8 @@ def load_core():
         lib.gpumpi_parse_args.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(Config)]
         lib.gpumpi_run.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                    vp, ctypes.POINTER(Result)]
+        lib.gpumpi_run_graph.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                         vp, ctypes.c_int, ctypes.POINTER(Result), vp]
         lib.gpumpi_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), vp]
         lib.gpumpi_device_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), sz]
         lib.gpumpi_device_free.argtypes = [vp]
@@ -121,6 +123,7 @@ def load():
         lib.gpumpi_verify_payload.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                               ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.gpumpi_verify_payload.restype = ctypes.c_longlong
+        lib.gpumpi_fill_normal.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         _progs = lib
     return core, _progs
 
@@ -193,11 +196,13 @@ class DeviceBuffer:
             pass
 
 
-def run(program, nranks, args=(), check=True, argv=(), **cfg_kw):
+def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, **cfg_kw):
     """Launch rank program `program` on `nranks` ranks.
 
     `args` are 64-bit words handed to the program (ints or DeviceBuffers,
-    which pass their device address).  Returns the Result.
+    which pass their device address).  With graph_replays > 0 the launch is
+    captured in a hipGraph and replayed that many times (Result.kernel_ms is
+    the mean replay time; Result.replay_ms the list).  Returns the Result.
     """
     core, progs = load()
     kern = progs.gpumpi_prog_lookup(program.encode())
@@ -215,7 +220,13 @@ def run(program, nranks, args=(), check=True, argv=(), **cfg_kw):
     dargs = DeviceBuffer.from_array(words)
     cargv = (ctypes.c_char_p * (len(argv) + 1))(*[s.encode() for s in argv], None)
     res = Result()
-    rc = core.gpumpi_run(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, ctypes.byref(res))
+    if graph_replays > 0:
+        ms = np.zeros(graph_replays, dtype=np.float64)
+        rc = core.gpumpi_run_graph(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, graph_replays,
+                                   ctypes.byref(res), ms.ctypes.data)
+        res.replay_ms = ms.tolist()
+    else:
+        rc = core.gpumpi_run(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, ctypes.byref(res))
     dargs.free()
     if check and rc != 0:
         raise GpumpiError(f"{program}: {res.describe()} :: {last_error()}")
@@ -226,6 +237,12 @@ def fill_payload(buf, stride, nbytes, seed, first_rank, nbuf):
     _, progs = load()
     if progs.gpumpi_fill_payload(buf.ptr, stride, nbytes, seed, first_rank, nbuf) != 0:
         raise GpumpiError("fill_payload failed")
+
+
+def fill_normal(buf, n, seed, stream, offset_elems=0):
+    _, progs = load()
+    if progs.gpumpi_fill_normal(buf.ptr + 4 * offset_elems, n, seed, stream) != 0:
+        raise GpumpiError("fill_normal failed")
 
 
 def verify_payload(buf, stride, nbytes, seed, shift, nranks, nbuf):

@@ -86,6 +86,13 @@ int gpumpi_parse_args(int argc, char** argv, gpumpi_config* cfg);
 int gpumpi_run(const gpumpi_config* cfg, const void* kernel, int argc, char** argv, void* user_args,
                gpumpi_result* result);
 
+/* Capture one launch of `kernel` into a hipGraph and replay it `replays`
+ * times (iteration loops of a whole MPI program without host launch work).
+ * Channel state carries over between replays.  replay_ms[i] (optional)
+ * receives each replay's HIP-event time; result->kernel_ms their mean. */
+int gpumpi_run_graph(const gpumpi_config* cfg, const void* kernel, int argc, char** argv, void* user_args,
+                     int replays, gpumpi_result* result, double* replay_ms);
+
 /* Drop-in host main: parse `---gpumpi`, run, return 0 or 1 like main.cu. */
 int gpumpi_main(int argc, char** argv, const void* kernel);
 
