@@ -1,0 +1,136 @@
+"""Compressed-wire Allreduce and the batched uneven Allreduce.
+
+Parity: the device quantisers equal the CPU oracle's BF16 / FP8-e4m3 codecs
+bit for bit, so the compressed result equals the oracle's rank-order FP32
+fold of the dequantised inputs exactly; the on-device FP64 error report of
+the batched program equals the CPU's FP64 computation; graph replays give
+the same results as a plain launch."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _deq_bf16(orc, x):
+    h = orc.bf16_encode(x).astype(np.uint32) << 16
+    return h.view(np.float32)
+
+
+def _deq_fp8(orc, x, block=256):
+    _, _, y = orc.e4m3_block_roundtrip(x, block)
+    return y
+
+
+def _fold_f32(rows):
+    acc = rows[0].astype(np.float32).copy()
+    for r in rows[1:]:
+        acc = (r.astype(np.float32) + acc).astype(np.float32)
+    return acc
+
+
+@pytest.mark.parametrize("wire", [1, 2])
+@pytest.mark.parametrize("P,n", [(4, 1000), (9, 70001)])
+def test_wire_allreduce_bit_exact(gm, orc, wire, P, n):
+    rng = np.random.default_rng(P * 7 + wire)
+    x = (rng.standard_normal((P, n)) * 10.0 ** rng.integers(-3, 3, (P, 1))).astype(np.float32)
+    stride = (n * 4 + 255) // 256 * 256
+    ib = gm.DeviceBuffer(stride * P)
+    ob = gm.DeviceBuffer(stride * P)
+    ob.zero()
+    for r in range(P):
+        ib.upload(x[r], offset=r * stride)
+    print(f"[gpu] wire={wire} P={P} n={n}", flush=True)
+    res = gm.run("coll_reduce", P, [0, gm.DT["MPI_FLOAT"], gm.OP["MPI_SUM"], n, ib, ob, stride, -1, 0, 1, 1, 0, 0,
+                                    wire], timeout_s=60.0)
+    assert res.status == 0, res.describe()
+    deq = np.stack([_deq_bf16(orc, x[r]) if wire == 1 else _deq_fp8(orc, x[r]) for r in range(P)])
+    exp = _fold_f32(deq)
+    for r in range(P):
+        got = ob.download(np.float32, n, offset=r * stride)
+        assert got.tobytes() == exp.tobytes(), r
+    ref = x.astype(np.float64).sum(axis=0)
+    err = np.linalg.norm(exp - ref) / np.linalg.norm(ref)
+    assert err < (1e-2 if wire == 1 else 6e-2)
+
+
+def _batched_layout(P, seed, nmin_log2=6, nmax_log2=12):
+    rng = np.random.default_rng(seed)
+    sizes = []
+    while sum(sizes) < P:
+        sizes.append(int(rng.integers(4, 65)))
+    excess = sum(sizes) - P
+    sizes[-1] -= excess
+    if sizes[-1] < 4:            # merge a too-small last group into the previous one
+        sizes[-2] += sizes[-1]
+        sizes.pop()
+    colors = np.repeat(np.arange(len(sizes)), sizes).astype(np.int32)
+    counts = np.floor(2.0 ** rng.uniform(nmin_log2, nmax_log2, len(sizes))).astype(np.int64)
+    offs = np.zeros(P, dtype=np.uint64)
+    o = 0
+    for r in range(P):
+        offs[r] = o
+        o += int(counts[colors[r]] + 63) // 64 * 64
+    return colors, counts, offs, o
+
+
+@pytest.mark.parametrize("wire", [0, 1, 2])
+def test_batched_allreduce_errors_match_cpu(gm, orc, wire):
+    P = 96
+    colors, counts, offs, total = _batched_layout(P, 5)
+    C = counts.size
+    inb = gm.DeviceBuffer(4 * total)
+    outb = gm.DeviceBuffer(4 * total)
+    refb = gm.DeviceBuffer(8 * total)
+    for r in range(P):
+        gm.fill_normal(inb, int(counts[colors[r]]), 77, r, offset_elems=int(offs[r]))
+    cb = gm.DeviceBuffer.from_array(colors)
+    nb = gm.DeviceBuffer.from_array(counts)
+    ob = gm.DeviceBuffer.from_array(offs)
+    tb = gm.DeviceBuffer(8 * P)
+    eb = gm.DeviceBuffer(16 * C)
+    eb.zero()
+    print(f"[gpu] batched wire={wire} comms={C}", flush=True)
+    res = gm.run("batched_allreduce", P, [0, cb, nb, inb, outb, ob, wire, 1, tb, eb, refb], timeout_s=120.0)
+    assert res.status == 0, res.describe()
+    errs = eb.download(np.float64).reshape(C, 2)
+    x = inb.download(np.float32)
+    y = outb.download(np.float32)
+    for c in range(C):
+        members = np.nonzero(colors == c)[0]
+        n = int(counts[c])
+        rows = np.stack([x[int(offs[r]):int(offs[r]) + n] for r in members])
+        ref = rows.astype(np.float64)[0].copy()
+        for row in rows[1:]:
+            ref = row.astype(np.float64) + ref      # FP64 rank-order fold
+        if wire == 0:
+            exp = orc.reduce_fold("MPI_FLOAT", "MPI_SUM", rows)
+        elif wire == 1:
+            exp = _fold_f32(np.stack([_deq_bf16(orc, row) for row in rows]))
+        else:
+            exp = _fold_f32(np.stack([_deq_fp8(orc, row) for row in rows]))
+        for r in members:
+            assert np.array_equal(y[int(offs[r]):int(offs[r]) + n], exp), (c, r)
+        s1 = float(np.sum((exp.astype(np.float64) - ref) ** 2))
+        s2 = float(np.sum(ref ** 2))
+        assert errs[c, 0] == pytest.approx(s1, rel=1e-9, abs=1e-300)
+        assert errs[c, 1] == pytest.approx(s2, rel=1e-12)
+        rel = np.sqrt(errs[c, 0] / errs[c, 1])
+        assert rel <= {0: 1e-6, 1: 1e-2, 2: 6e-2}[wire]
+
+
+def test_graph_replays_match_plain_launch(gm, orc):
+    P, n = 8, 5000
+    rng = np.random.default_rng(9)
+    x = rng.integers(-1000, 1000, (P, n)).astype(np.int32)
+    stride = n * 4
+    ib = gm.DeviceBuffer.from_array(x)
+    ob = gm.DeviceBuffer(stride * P)
+    ob.zero()
+    res = gm.run("coll_reduce", P, [0, gm.DT["MPI_INT"], gm.OP["MPI_SUM"], n, ib, ob, stride, -1, 0, 1, 2, 0, 0],
+                 graph_replays=5, timeout_s=60.0)
+    assert res.status == 0, res.describe()
+    assert len(res.replay_ms) == 5 and min(res.replay_ms) > 0
+    exp = orc.reduce_fold("MPI_INT", "MPI_SUM", x)
+    got = ob.download(np.int32).reshape(P, n)
+    for r in range(P):
+        assert np.array_equal(got[r], exp)
