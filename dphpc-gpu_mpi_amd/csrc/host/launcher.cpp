@@ -562,6 +562,11 @@ int gpumpi_main(int argc, char** argv, const void* kernel) {
     return r.any_rank_failed ? 1 : 0;
 }
 
+int gpumpi_reset_world(void) {
+    g_world.need_reset = true;
+    return 0;
+}
+
 int gpumpi_device_alloc(void** ptr, size_t bytes) {
     HIPOK(hipMalloc(ptr, bytes ? bytes : 1));
     return 0;

@@ -162,11 +162,27 @@ GPUMPI_PROGRAM(ka_all_to_all_fifo) {
     MPI_Waitall(reps * size, sr, MPI_STATUSES_IGNORE);
     MPI_Waitall(reps * size, rr, MPI_STATUSES_IGNORE);
     bool ok = true;
+    int64_t* diag = prog_ptr<int64_t>(1);  // optional: [rank*64]: count, then (k, r, i, got) records
+    int nbad = 0;
     for (int k = 0; k < reps; k++)
         for (int r = 0; r < size; r++) {
             if (r == rank) continue;
-            for (int i = 0; i < n; i++) ok = ok && remote[i + k * n + r * n * reps] == i * k * rank;
+            for (int i = 0; i < n; i++) {
+                const int got = remote[i + k * n + r * n * reps];
+                if (got != i * k * rank) {
+                    ok = false;
+                    if (diag && nbad < 15) {
+                        int64_t* d = diag + (int64_t)rank * 64 + 1 + 4 * nbad;
+                        d[0] = k;
+                        d[1] = r;
+                        d[2] = i;
+                        d[3] = got;
+                    }
+                    nbad++;
+                }
+            }
         }
+    if (diag) diag[(int64_t)rank * 64] = nbad;
     MPI_Finalize();
     set_ok(ok);
     return 0;

@@ -35,7 +35,8 @@ OP = {n: i for i, n in enumerate(OPS)}
 # Exported symbols of each library (the C-ABI surface).
 CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpumpi_run_graph", "gpumpi_main",
                 "gpumpi_device_alloc", "gpumpi_device_free", "gpumpi_memcpy_htod",
-                "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error"]
+                "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error",
+                "gpumpi_reset_world"]
 PROGS_SYMBOLS = ["gpumpi_prog_lookup", "gpumpi_prog_count", "gpumpi_prog_name",
                  "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal"]
 
@@ -126,6 +127,10 @@ def load():
         lib.gpumpi_fill_normal.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         _progs = lib
     return core, _progs
+
+
+def reset_world():
+    load_core().gpumpi_reset_world()
 
 
 def last_error():

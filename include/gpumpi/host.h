@@ -104,6 +104,10 @@ int gpumpi_memcpy_htod(void* dst, const void* src, size_t bytes);
 int gpumpi_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int gpumpi_memset(void* dst, int value, size_t bytes);
 
+/* Start the next launch from fresh channel state (by default channel
+ * cursors carry over between clean launches of the same world shape). */
+int gpumpi_reset_world(void);
+
 /* Largest rank count that can be co-resident for a configuration. */
 int gpumpi_max_ranks(const gpumpi_config* cfg, const void* kernel);
 
