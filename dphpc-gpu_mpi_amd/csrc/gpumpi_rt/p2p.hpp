@@ -292,6 +292,19 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
     if (err != MPI_SUCCESS) GPUMPI_FAIL(err, "message truncated: receive buffer too small");
 }
 
+// Slots of one channel are read by different lanes with independent loads,
+// so a later slot can be seen filled while an earlier one (written first by
+// the sender) still reads as empty.  Matching is therefore confined to the
+// prefix of slots seen arrived or already consumed: everything past the
+// first slot not yet seen waits for the next poll, which keeps the
+// non-overtaking order of MPI.  `group` is the ballot of arrived-or-consumed
+// slots of this channel (bit j = slot head+j).
+__device__ __forceinline__ uint64_t arrived_prefix(uint64_t group, int Q) {
+    const uint64_t all = Q >= 64 ? ~0ull : ((1ull << Q) - 1);
+    const uint64_t missing = ~group & all;
+    return missing ? ((1ull << __builtin_ctzll(missing)) - 1) : all;
+}
+
 // Look for the oldest message of channel src->me matching request idx.
 __device__ inline bool match_from(int idx, int src) {
     RankLds* L = ctx();
@@ -299,14 +312,18 @@ __device__ inline bool match_from(int idx, int src) {
     const int Q = L->Q;
     const RxChan rc = rxc()[src];
     const int j = lane_id();
-    bool ok = false;
+    bool ok = false, seen = false;
     uint64_t env = 0;
-    if (j < Q && !((rc.mask >> j) & 1ull)) {
-        env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, rc.head + (uint32_t)j))->env);
-        ok = env_seq(env) == rc.head + (uint32_t)j + 1u && env_ctx(env) == q->ctx &&
-             (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
+    if (j < Q) {
+        if ((rc.mask >> j) & 1ull) {
+            seen = true;
+        } else {
+            env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, rc.head + (uint32_t)j))->env);
+            seen = env_seq(env) == rc.head + (uint32_t)j + 1u;
+            ok = seen && env_ctx(env) == q->ctx && (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
+        }
     }
-    const uint64_t b = ballot(ok);
+    const uint64_t b = ballot(ok) & arrived_prefix(ballot(seen), Q);
     if (!b) return false;
     const int first = __builtin_ctzll(b);
     env = uniform_u64(__shfl(env, first));
@@ -315,7 +332,8 @@ __device__ inline bool match_from(int idx, int src) {
 }
 
 // MPI_ANY_SOURCE: scan the channels of every member of the communicator,
-// lowest member first.
+// lowest member first.  Q divides 64, so each channel's Q slots fall in one
+// 64-lane batch (lanes [g*Q, g*Q+Q) of it).
 __device__ inline bool match_any(int idx) {
     RankLds* L = ctx();
     const Req* q = &reqs()[idx];
@@ -323,9 +341,11 @@ __device__ inline bool match_any(int idx) {
     const int size = c->size;
     const int Q = L->Q;
     const int total = size * Q;
+    const int lane = lane_id();
+    const int gbase = lane - lane % Q;  // first lane of this lane's channel
     for (int base = 0; base < total; base += 64) {
-        const int k = base + lane_id();
-        bool ok = false;
+        const int k = base + lane;
+        bool ok = false, seen = false;
         uint64_t env = 0;
         int src = 0;
         uint32_t seq = 0;
@@ -334,12 +354,17 @@ __device__ inline bool match_any(int idx) {
             src = comm_world_of(c, r);
             const RxChan rc = rxc()[src];
             seq = rc.head + (uint32_t)j;
-            if (!((rc.mask >> j) & 1ull)) {
+            if ((rc.mask >> j) & 1ull) {
+                seen = true;
+            } else {
                 env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, seq))->env);
-                ok = env_seq(env) == seq + 1u && env_ctx(env) == q->ctx &&
-                     (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
+                seen = env_seq(env) == seq + 1u;
+                ok = seen && env_ctx(env) == q->ctx && (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
             }
         }
+        const uint64_t seen_b = ballot(seen);
+        const uint64_t pre = arrived_prefix(Q >= 64 ? seen_b : (seen_b >> gbase), Q);
+        ok = ok && ((pre >> (lane - gbase)) & 1ull);
         const uint64_t b = ballot(ok);
         if (b) {
             const int first = __builtin_ctzll(b);

@@ -225,6 +225,10 @@ def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, **cfg_kw
     dargs = DeviceBuffer.from_array(words)
     cargv = (ctypes.c_char_p * (len(argv) + 1))(*[s.encode() for s in argv], None)
     res = Result()
+    trace = os.environ.get("GPUMPI_TRACE")
+    if trace:
+        print(f"[gpumpi] launch {program} P={nranks} threads={cfg.threads_per_rank} "
+              f"replays={graph_replays} args={[int(w) for w in words[:len(args)]]}", flush=True)
     if graph_replays > 0:
         ms = np.zeros(graph_replays, dtype=np.float64)
         rc = core.gpumpi_run_graph(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, graph_replays,
@@ -233,6 +237,8 @@ def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, **cfg_kw
     else:
         rc = core.gpumpi_run(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, ctypes.byref(res))
     dargs.free()
+    if trace:
+        print(f"[gpumpi] done {program} rc={rc} status={res.status}", flush=True)
     if check and rc != 0:
         raise GpumpiError(f"{program}: {res.describe()} :: {last_error()}")
     return res
