@@ -141,7 +141,7 @@ __device__ inline bool gpumpi_is_leader_lane() { return gpumpi::rt::elected(); }
 // launch configuration); shared by every wave of the rank.
 template <typename T>
 __device__ inline T* gpumpi_user_lds() {
-    return (T*)(gpumpi::rt::gpumpi_lds_base + gpumpi::rt::ctx()->off_user);
+    return (T*)((uint8_t*)gpumpi::rt::ctx() + gpumpi::rt::ctx()->off_user);
 }
 
 // Run `f(t, nt)` on every thread t of the rank's workgroup (nt threads,

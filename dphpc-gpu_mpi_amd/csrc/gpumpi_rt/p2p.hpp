@@ -305,8 +305,15 @@ __device__ __forceinline__ uint64_t arrived_prefix(uint64_t group, int Q) {
     return missing ? ((1ull << __builtin_ctzll(missing)) - 1) : all;
 }
 
+// A message found for a receive: channel `src`, sequence `seq`, envelope.
+struct Found {
+    int src;
+    uint32_t seq;
+    uint64_t env;
+};
+
 // Look for the oldest message of channel src->me matching request idx.
-__device__ inline bool match_from(int idx, int src) {
+__device__ inline bool match_from(int idx, int src, Found* f) {
     RankLds* L = ctx();
     const Req* q = &reqs()[idx];
     const int Q = L->Q;
@@ -326,15 +333,16 @@ __device__ inline bool match_from(int idx, int src) {
     const uint64_t b = ballot(ok) & arrived_prefix(ballot(seen), Q);
     if (!b) return false;
     const int first = __builtin_ctzll(b);
-    env = uniform_u64(__shfl(env, first));
-    deliver(idx, src, rc.head + (uint32_t)first, env);
+    f->src = src;
+    f->seq = rc.head + (uint32_t)first;
+    f->env = uniform_u64(__shfl(env, first));
     return true;
 }
 
 // MPI_ANY_SOURCE: scan the channels of every member of the communicator,
 // lowest member first.  Q divides 64, so each channel's Q slots fall in one
 // 64-lane batch (lanes [g*Q, g*Q+Q) of it).
-__device__ inline bool match_any(int idx) {
+__device__ inline bool match_any(int idx, Found* f) {
     RankLds* L = ctx();
     const Req* q = &reqs()[idx];
     CommShared* c = comm_get((MPI_Comm)(uintptr_t)q->comm);
@@ -368,19 +376,49 @@ __device__ inline bool match_any(int idx) {
         const uint64_t b = ballot(ok);
         if (b) {
             const int first = __builtin_ctzll(b);
-            env = uniform_u64(__shfl(env, first));
-            src = uniform_i32(__shfl(src, first));
-            seq = uniform_u32(__shfl((int)seq, first));
-            deliver(idx, src, seq, env);
+            f->env = uniform_u64(__shfl(env, first));
+            f->src = uniform_i32(__shfl(src, first));
+            f->seq = uniform_u32(__shfl((int)seq, first));
             return true;
         }
     }
     return false;
 }
 
-__device__ __forceinline__ bool try_match(int idx) {
+__device__ __forceinline__ bool find_match(int idx, Found* f) {
     const Req* q = &reqs()[idx];
-    return q->peer >= 0 ? match_from(idx, q->peer) : match_any(idx);
+    return q->peer >= 0 ? match_from(idx, q->peer, f) : match_any(idx, f);
+}
+
+// Match and deliver request idx (used when no earlier posted receive could
+// take the same message).
+__device__ __forceinline__ bool try_match(int idx) {
+    Found f;
+    if (!find_match(idx, &f)) return false;
+    deliver(idx, f.src, f.seq, f.env);
+    return true;
+}
+
+// The earliest pending receive posted before ring position `pos` that would
+// also accept message f (same source or MPI_ANY_SOURCE, same context, tag
+// or MPI_ANY_TAG), or -1.  A message that arrives while a progress pass is
+// between an earlier receive and a later one must still go to the earlier
+// one (MPI's matching order of posted receives).
+__device__ inline int earlier_taker(int pos, const Found& f) {
+    RankLds* L = ctx();
+    // The receive that found f is one of the unmatched receives counted here.
+    if (L->unmatched_any + (int)unmatched_src()[f.src] < 2) return -1;
+    const int ctxid = env_ctx(f.env), tag = env_tag(f.env);
+    for (int i = L->ring_head; i < pos; i++) {
+        int oi;
+        if (!ring_live(i, &oi)) continue;
+        const Req* o = &reqs()[oi];
+        if (o->kind != REQ_RECV || o->state != RS_PENDING) continue;
+        if ((o->peer != f.src && o->peer >= 0) || o->ctx != ctxid) continue;
+        if (o->tag != MPI_ANY_TAG && o->tag != tag) continue;
+        return oi;
+    }
+    return -1;
 }
 
 // One pass over every active request in posting order.
@@ -409,7 +447,11 @@ __device__ inline void progress() {
                 }
             }
         } else if (q->kind == REQ_RECV && q->state == RS_PENDING) {
-            try_match(idx);
+            Found f;
+            if (!find_match(idx, &f)) continue;
+            const int e = earlier_taker(i, f);
+            if (e < 0) deliver(idx, f.src, f.seq, f.env);
+            else try_match(e);  // takes its own oldest match: f or an older one
         }
     }
 }

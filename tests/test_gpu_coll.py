@@ -213,7 +213,9 @@ def test_cpi(gm, orc, P, n):
     pi_ref, parts_ref = orc.pi_fold(n, P)
     assert np.array_equal(got[2:], parts_ref)  # FP64 partial sums bit-exact
     assert got[0] == pi_ref                   # rank-order fold bit-exact
-    assert abs(got[0] - math.pi) < 2e-12
+    # midpoint rule: |pi_n - pi| = h^2/12 to leading order (h = 1/n), then FP64 rounding
+    h = 1.0 / n
+    assert abs(got[0] - math.pi) <= h * h / 12 * 1.001 + 2e-12
     if n == 100000000:
         gold = json.load(open(os.path.join(GOLDEN, "cpi_mpich.json")))
         assert abs(got[0] - float(gold["pi_printed"][str(P)])) / math.pi <= 1e-13
