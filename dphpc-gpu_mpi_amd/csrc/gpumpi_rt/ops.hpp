@@ -82,7 +82,12 @@ struct FSum {
     __device__ __forceinline__ cdouble2 operator()(cdouble2 a, cdouble2 b) const { return {a.re + b.re, a.im + b.im}; }
 };
 struct FProd {
-    template <class T> __device__ __forceinline__ T operator()(T a, T b) const { return a * b; }
+    // 8- and 16-bit operands would be promoted to (signed) int, whose
+    // overflow is undefined; multiply them as unsigned int and truncate.
+    template <class T> __device__ __forceinline__ T operator()(T a, T b) const {
+        if constexpr (sizeof(T) < sizeof(unsigned) && T(-1) > T(0)) return (T)((unsigned)a * (unsigned)b);
+        else return a * b;
+    }
     __device__ __forceinline__ cfloat2 operator()(cfloat2 a, cfloat2 b) const {
 #pragma clang fp contract(off)
         return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
