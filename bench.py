@@ -60,6 +60,19 @@ def cpu_baseline(nbytes, budget_s=20.0):
     }
 
 
+def measured_traffic(P, S, threads, steps):
+    """HBM bytes per timed launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs recorded in profiles/ (per ring step, calibrated against a plain
+    copy kernel), when they were taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "p2p_ring_traffic.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    if (rec.get("ranks"), rec.get("message_bytes"), rec.get("threads")) != (P, S, threads):
+        return None
+    return round(rec["hbm_bytes_per_step"] * steps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,17 +100,20 @@ def main():
     rb.zero()
     gm.fill_payload(sb, stride, S, SEED, 0, P)
 
-    def ring(iters, warm):
+    def ring(program, iters):
         sz = gm.DeviceBuffer.from_array(np.array([S], dtype=np.uint64))
         it = gm.DeviceBuffer.from_array(np.array([iters], dtype=np.int64))
-        wm = gm.DeviceBuffer.from_array(np.array([warm], dtype=np.int64))
+        wm = gm.DeviceBuffer.from_array(np.array([0], dtype=np.int64))
         times = gm.DeviceBuffer(8 * P)
-        res = gm.run("p2p_ring_sweep", P, [0, sz, 1, it, wm, sb, rb, stride, times],
+        res = gm.run(program, P, [0, sz, 1, it, wm, sb, rb, stride, times],
                      threads_per_rank=args.threads, timeout_s=600.0)
         return res, times.download(np.float64)
 
-    ring(args.warmup, 0)                       # untimed warm-up launch
-    res, t = ring(args.steps, 0)               # timed launch: exactly K steps
+    # Warm-up launch under its own kernel name (p2p_ring_warmup), then the
+    # timed launch of exactly K steps (p2p_ring_sweep): a kernel trace of this
+    # command lists the timed launch alone under p2p_ring_sweep.
+    ring("p2p_ring_warmup", args.warmup)
+    res, t = ring("p2p_ring_sweep", args.steps)
     kernel_s = res.kernel_ms / 1e3
     bytes_per_step = 2.0 * P * S
     gbps = bytes_per_step * args.steps / kernel_s / 1e9
@@ -131,7 +147,8 @@ def main():
                                f"x {S} B per rank per step, MPI_Irecv+MPI_Isend+MPI_Waitall",
                    "ranks": P, "message_bytes": S, "parallelism": "1 GPU, 1 persistent kernel"},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic(P, S, args.threads, args.steps)},
         "cpu_baseline": cpu,
         "p2p_latency_us_8B_one_way": round(rtt_us / 2, 3),
         "p2p_rtt_us_8B": round(rtt_us, 3),

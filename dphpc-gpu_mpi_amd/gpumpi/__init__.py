@@ -38,7 +38,8 @@ CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpu
                 "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error",
                 "gpumpi_reset_world"]
 PROGS_SYMBOLS = ["gpumpi_prog_lookup", "gpumpi_prog_count", "gpumpi_prog_name",
-                 "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal"]
+                 "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal",
+                 "gpumpi_copy_calibration"]
 
 STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ETOOBIG", 4: "EDEVICE", 5: "ETIMEOUT", 6: "ENOMEM"}
 
@@ -125,6 +126,7 @@ def load():
                                               ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.gpumpi_verify_payload.restype = ctypes.c_longlong
         lib.gpumpi_fill_normal.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+        lib.gpumpi_copy_calibration.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
         _progs = lib
     return core, _progs
 
@@ -254,6 +256,14 @@ def fill_normal(buf, n, seed, stream, offset_elems=0):
     _, progs = load()
     if progs.gpumpi_fill_normal(buf.ptr + 4 * offset_elems, n, seed, stream) != 0:
         raise GpumpiError("fill_normal failed")
+
+
+def copy_calibration(dst, src, nbytes):
+    """Plain device copy of nbytes (multiple of 16): the reference kernel for
+    HBM traffic counters."""
+    _, progs = load()
+    if progs.gpumpi_copy_calibration(dst.ptr, src.ptr, nbytes) != 0:
+        raise GpumpiError("copy_calibration failed")
 
 
 def verify_payload(buf, stride, nbytes, seed, shift, nranks, nbuf):
