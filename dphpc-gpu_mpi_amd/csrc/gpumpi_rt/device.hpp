@@ -83,6 +83,7 @@ enum ReqState : uint8_t {
     RS_PENDING = 0,   // send: no slot yet; recv: not matched yet
     RS_WAIT_DONE = 1, // rendezvous send posted, receiver not done copying
     RS_COMPLETE = 2,
+    RS_MATCHED = 3,   // recv matched to a rendezvous message; the copy is still to do
 };
 enum ReqFlags : uint8_t { RF_SYNC = 1, RF_STAGED = 2, RF_PRIVATE_DST = 4, RF_PROC_NULL = 8 };
 
@@ -103,7 +104,9 @@ struct Req {
     int32_t st_error;
     uint32_t stamp;     // posting order stamp; also the rendezvous done value
     int32_t ringpos;    // position in the posting-order ring (validates entries)
-    int32_t pad2;
+    uint32_t rv_val;    // matched rendezvous: value for the sender's done word
+    uint64_t rv_src;    // matched rendezvous: sender's buffer
+    uint64_t rv_done;   // matched rendezvous: sender's done word
 };
 
 struct RxChan {   // receiver view of channel src -> me
@@ -157,6 +160,7 @@ struct RankLds {
     int ring_head, ring_tail;
     int unmatched_any;   // unmatched MPI_ANY_SOURCE receives
     int pending_sends;   // sends still waiting for a channel slot
+    int pending_copies;  // matched rendezvous receives whose copy is still to do
     uint64_t bounce_top; // staging bump offset for private rendezvous sends
     int bounce_users;
     // offsets of the variable arrays from the LDS base

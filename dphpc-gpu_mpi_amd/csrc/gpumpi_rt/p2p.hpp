@@ -80,7 +80,7 @@ __device__ inline void ring_compact() {
     L->ring_tail = out;
 }
 
-__device__ inline void progress();
+__device__ inline void progress(bool copies = true);
 
 __device__ inline int req_alloc() {
     RankLds* L = ctx();
@@ -112,7 +112,7 @@ __device__ inline int req_alloc() {
             }
         }
         if (attempt > 64 || aborted()) break;
-        progress();
+        progress(false);
     }
     GPUMPI_FAIL(MPI_ERR_REQUEST, "request pool exhausted (raise pending requests, -e)");
     return -1;
@@ -260,12 +260,14 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
         else
             copy_part(dst, (uint8_t*)hdr + kSlotHeaderBytes, n, lane_id(), 64);
     } else {
-        const void* from = (const void*)(uintptr_t)uniform_u64(hdr->src_addr);
-        uint32_t* done = (uint32_t*)(uintptr_t)uniform_u64(hdr->done_addr);
-        const uint32_t dval = uniform_u32(hdr->done_val);
-        rank_copy((void*)(uintptr_t)q->buf, from, n);
-        fence_release_agent();
-        if (elected()) st_relaxed_agent(done, dval);
+        // Rendezvous: only the match happens here.  The copy out of the
+        // sender's buffer runs later, from a wait or test call
+        // (finish_rendezvous), so that an MPI_Irecv that finds its message
+        // already announced returns at once and the rank's own sends get
+        // posted before it spends time copying.
+        q->rv_src = uniform_u64(hdr->src_addr);
+        q->rv_done = uniform_u64(hdr->done_addr);
+        q->rv_val = uniform_u32(hdr->done_val);
     }
     // Mark consumed; publish the new head once the oldest slots are done.
     RxChan* rc = &rxc()[src];
@@ -282,7 +284,8 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
         fence_release_agent();
         if (elected()) st_relaxed_agent(&world()->chan_head[(uint64_t)me * L->nranks + src], h);
     }
-    q->state = RS_COMPLETE;
+    q->state = (flags & SLOT_EAGER) ? RS_COMPLETE : RS_MATCHED;
+    if (!(flags & SLOT_EAGER)) L->pending_copies++;
     q->got = n;
     q->st_source = src;
     q->st_tag = env_tag(env);
@@ -421,8 +424,23 @@ __device__ inline int earlier_taker(int pos, const Found& f) {
     return -1;
 }
 
-// One pass over every active request in posting order.
-__device__ inline void progress() {
+// Copy a matched rendezvous message out of the sender's buffer (every wave
+// of the rank), then release the sender's done word.
+__device__ inline void finish_rendezvous(int idx) {
+    RankLds* L = ctx();
+    Req* q = &reqs()[idx];
+    rank_copy((void*)(uintptr_t)q->buf, (const void*)(uintptr_t)q->rv_src, q->got);
+    fence_release_agent();
+    if (elected()) st_relaxed_agent((uint32_t*)(uintptr_t)q->rv_done, q->rv_val);
+    q->state = RS_COMPLETE;
+    L->pending_copies--;
+}
+
+// One pass over every active request in posting order: post waiting sends,
+// collect rendezvous completions, match receives.  With `copies` (wait and
+// test calls) a second pass then performs the copies of matched rendezvous
+// receives, so every send this rank can post is out before it copies.
+__device__ inline void progress(bool copies) {
     RankLds* L = ctx();
     const int R = L->R;
     uint64_t blocked = 0;  // destinations (hashed) whose earlier send found no slot
@@ -453,6 +471,13 @@ __device__ inline void progress() {
             if (e < 0) deliver(idx, f.src, f.seq, f.env);
             else try_match(e);  // takes its own oldest match: f or an older one
         }
+    }
+    if (!copies || L->pending_copies == 0) return;
+    for (int i = L->ring_head; i < L->ring_tail; i++) {
+        int idx;
+        if (!ring_live(i, &idx)) continue;
+        const Req* q = &reqs()[idx];
+        if (q->kind == REQ_RECV && q->state == RS_MATCHED) finish_rendezvous(idx);
     }
 }
 
@@ -486,7 +511,7 @@ __device__ inline int start_recv(void* buf, uint64_t bytes, int src, int tag, in
                          (src < 0 && L->ring_tail - L->ring_head > 1);
     if (src < 0) L->unmatched_any++;
     else unmatched_src()[src] = (uint16_t)(unmatched_src()[src] + 1);
-    if (earlier) progress();
+    if (earlier) progress(false);
     else try_match(idx);
     return idx;
 }
