@@ -11,6 +11,29 @@ namespace rt {
 
 __device__ void run_cmd(const Cmd& c, int t, int nt);  // coll.hpp
 
+// Helper-wave handoff: post `c` for waves 1..W-1 (returns the sequence to
+// wait for), and wait until they have all acknowledged it.
+__device__ inline uint32_t wg_post(const Cmd& c) {
+    RankLds* L = ctx();
+    const uint32_t seq = L->cmd_seq + 1;
+    L->cmd = c;
+    fence_release_wg();
+    __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return seq;
+}
+__device__ inline void wg_wait(uint32_t seq) {
+    RankLds* L = ctx();
+    const uint32_t target = seq * (uint32_t)(L->waves - 1);
+    int n = 0;
+    while (__hip_atomic_load(&L->ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++n >= (1 << 20)) {
+            n = 0;
+            if (aborted()) break;
+        }
+    }
+}
+
 // Run `c` on every wave of this rank (wave 0 included) and return when all
 // waves are done with it.
 __device__ inline void wg_run(const Cmd& c) {
@@ -20,20 +43,9 @@ __device__ inline void wg_run(const Cmd& c) {
         run_cmd(c, lane_id(), 64);
         return;
     }
-    const uint32_t seq = L->cmd_seq + 1;
-    L->cmd = c;
-    fence_release_wg();
-    __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t seq = wg_post(c);
     run_cmd(c, (int)threadIdx.x, waves * 64);
-    const uint32_t target = seq * (uint32_t)(waves - 1);
-    int n = 0;
-    while (__hip_atomic_load(&L->ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++n >= (1 << 20)) {
-            n = 0;
-            if (aborted()) break;
-        }
-    }
+    wg_wait(seq);
 }
 
 // A device functor run by every thread of the rank (CMD_USER): the functor
@@ -71,12 +83,17 @@ __device__ inline void rank_copy(void* dst, const void* src, uint64_t n) {
         copy_part(dst, src, n, lane_id(), 64);
         return;
     }
+    // Wave 0 runs its share of the copy inline (not through run_cmd), so the
+    // point-to-point path that calls this stays free of the large command
+    // dispatcher and its register demands.
     Cmd c{};
     c.op = CMD_COPY;
     c.a[0] = (uint64_t)(uintptr_t)dst;
     c.a[1] = (uint64_t)(uintptr_t)src;
     c.a[2] = n;
-    wg_run(c);
+    const uint32_t seq = wg_post(c);
+    copy_part(dst, src, n, (int)threadIdx.x, ctx()->waves * 64);
+    wg_wait(seq);
 }
 
 // Body of waves 1..W-1.

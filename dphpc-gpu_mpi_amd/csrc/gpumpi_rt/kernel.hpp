@@ -75,7 +75,7 @@ __device__ inline void rank_save(World* w) {
     bool stale = false;
     for (int src = lane_id(); src < ctx()->nranks; src += 64) {
         const uint32_t h = rxc()[src].head;
-        const uint64_t e = ld_relaxed_agent(&((const SlotHdr*)slot_at(my_rank(), src, h))->env);
+        const uint64_t e = ld_relaxed_agent((const uint64_t*)slot_at(my_rank(), src, h));
         if (env_seq(e) == h + 1) stale = true;
     }
     if (ballot(stale) && elected())

@@ -151,6 +151,30 @@ __device__ inline uint8_t* bounce_alloc(uint64_t bytes) {
 // ---------------------------------------------------------------------------
 // Send side
 // ---------------------------------------------------------------------------
+// Payload word k of an n-byte buffer (bytes [4k, 4k+4), zero padded past n).
+__device__ __forceinline__ uint32_t slot_payload_get(const uint8_t* p, uint64_t n, uint64_t k) {
+    const uint64_t o = 4 * k;
+    if (o + 4 <= n && ((uintptr_t)p & 3) == 0) return *(const uint32_t*)(p + o);
+    uint32_t v = 0;
+    for (int b = 0; b < 4 && o + (uint64_t)b < n; b++) v |= (uint32_t)p[o + b] << (8 * b);
+    return v;
+}
+// Store payload word k into an n-byte buffer (bytes past n are dropped).
+__device__ __forceinline__ void slot_payload_put(uint8_t* p, uint64_t n, uint64_t k, uint32_t v) {
+    const uint64_t o = 4 * k;
+    if (o + 4 <= n && ((uintptr_t)p & 3) == 0) {
+        *(uint32_t*)(p + o) = v;
+        return;
+    }
+    for (int b = 0; b < 4 && o + (uint64_t)b < n; b++) p[o + b] = (uint8_t)(v >> (8 * b));
+}
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int l) { return (uint32_t)__shfl((int)v, l); }
+
+// Post send request idx into its channel if a slot is free.  Every word of
+// the slot (envelope, info, payload or rendezvous fields) is one 8-byte
+// store tagged with seq+1, written by the lanes in parallel with no fence:
+// the receiver validates each word by its tag.  A rendezvous send first
+// releases its buffer, which the receiver will read directly.
 __device__ inline bool try_post_send(int idx) {
     RankLds* L = ctx();
     Req* q = &reqs()[idx];
@@ -159,28 +183,46 @@ __device__ inline bool try_post_send(int idx) {
     const uint32_t s = tc->seq;
     const uint32_t Q = (uint32_t)L->Q;
     if (s - tc->head_cache >= Q) {
-        const uint32_t h = uniform_u32(ld_acquire_agent(&world()->chan_head[(uint64_t)dst * L->nranks + me]));
+        // The receiver publishes its head only after it has read the slots
+        // it consumed, so no fence is needed before reusing them.
+        const uint32_t h = uniform_u32(ld_relaxed_agent(&world()->chan_head[(uint64_t)dst * L->nranks + me]));
         tc->head_cache = h;
         if (s - h >= Q) return false;
     }
-    uint8_t* slot = slot_at(dst, me, s);
-    SlotHdr* hdr = (SlotHdr*)slot;
-    const bool eager = (q->bytes <= (uint64_t)L->eager_bytes) && !(q->flags & RF_SYNC);
+    uint64_t* slot = (uint64_t*)slot_at(dst, me, s);
+    const uint32_t seq1 = s + 1;
+    const uint64_t n = q->bytes;
+    const bool eager = n <= (uint64_t)L->eager_bytes && !(q->flags & RF_SYNC);
+    const uint32_t envv = slot_word_value(pack_env(0, q->ctx, q->tag));
+    const int lane = lane_id();
     if (eager) {
-        copy_part(slot + kSlotHeaderBytes, (const void*)(uintptr_t)q->buf, q->bytes, lane_id(), 64);
+        const uint8_t* src = (const uint8_t*)(uintptr_t)q->buf;
+        const int nw = eager_words(n);
+        for (int w = lane; w < nw; w += 64) {
+            uint32_t v;
+            if (w == SF_ENV) v = envv;
+            else if (w == SF_INFO) v = (SLOT_EAGER << 24) | (uint32_t)n;
+            else v = slot_payload_get(src, n, (uint64_t)(w - SF_PAYLOAD));
+            st_relaxed_agent(&slot[w], slot_word(seq1, v));
+        }
+    } else {
+        fence_release_agent();
+        const uint64_t done = (uint64_t)(uintptr_t)&world()->done_words[(uint64_t)me * L->R + idx];
+        if (lane < SF_RNDV_WORDS) {
+            uint32_t v = 0;
+            switch (lane) {
+            case SF_ENV: v = envv; break;
+            case SF_INFO: v = (SLOT_RNDV << 24) | (uint32_t)(n & 0xffffffu); break;
+            case SF_SRC_LO: v = (uint32_t)q->buf; break;
+            case SF_SRC_HI: v = (uint32_t)(q->buf >> 32); break;
+            case SF_DONE_LO: v = (uint32_t)done; break;
+            case SF_DONE_HI: v = (uint32_t)(done >> 32); break;
+            case SF_DONE_VAL: v = q->stamp; break;
+            default: v = (uint32_t)(n >> 24); break;
+            }
+            st_relaxed_agent(&slot[lane], slot_word(seq1, v));
+        }
     }
-    hdr->bytes = q->bytes;
-    hdr->flags = eager ? SLOT_EAGER : SLOT_RNDV;
-    hdr->src_world = (uint32_t)me;
-    if (!eager) {
-        hdr->src_addr = q->buf;
-        hdr->done_addr = (uint64_t)(uintptr_t)&world()->done_words[(uint64_t)me * L->R + idx];
-        hdr->done_val = q->stamp;
-    }
-    // Envelope last: the release fence orders the payload and header (every
-    // lane's stores) before it.
-    fence_release_agent();
-    if (elected()) st_relaxed_agent(&hdr->env, pack_env(s + 1, q->ctx, q->tag));
     tc->seq = s + 1;
     q->state = eager ? RS_COMPLETE : RS_WAIT_DONE;
     return true;
@@ -238,38 +280,81 @@ __device__ inline int start_send(const void* buf, uint64_t bytes, int dst, int t
 // ---------------------------------------------------------------------------
 // Receive side
 // ---------------------------------------------------------------------------
+// Make lane l's word (index base + l, loaded into v when loaded) valid for
+// every word below `need`: re-load those that do not carry tag seq1 yet.
+// The envelope was seen already and the other words were stored by the
+// same sender instructions, so this waits at most for their propagation.
+__device__ inline uint64_t slot_validate(const uint64_t* slot, int base, int need, uint32_t seq1, uint64_t v) {
+    const int w = base + lane_id();
+    for (;;) {
+        const bool bad = w < need && slot_word_seq(v) != seq1;
+        if (ballot(bad) == 0) return v;
+        if (bad) v = ld_relaxed_agent(&slot[w]);
+    }
+}
+
 // Deliver message `seq` of channel src->me to receive request `idx`.
 __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
     RankLds* L = ctx();
     Req* q = &reqs()[idx];
     const int me = L->rank;
-    fence_acquire_agent();
-    SlotHdr* hdr = (SlotHdr*)slot_at(me, src, seq);
-    const uint64_t bytes = uniform_u64(hdr->bytes);
-    const uint32_t flags = uniform_u32(hdr->flags);
+    const uint64_t* slot = (const uint64_t*)slot_at(me, src, seq);
+    const uint32_t seq1 = seq + 1;
+    const int swords = L->slot_bytes / 8;
+    // Words 0..63 (at most the slot) in one round trip; the info word then
+    // says how many of them (and of later chunks) the message uses.
+    const int first = swords < 64 ? swords : 64;
+    uint64_t v = lane_id() < first ? ld_relaxed_agent(&slot[lane_id()]) : 0;
+    v = slot_validate(slot, 0, SF_INFO + 1, seq1, v);
+    const uint32_t info = lane_u32(slot_word_value(v), SF_INFO);
+    const uint32_t flags = info >> 24;
+    const bool eager = (flags & SLOT_EAGER) != 0;
+    const int need = eager ? eager_words(info & 0xffffffu) : (int)SF_RNDV_WORDS;
+    v = slot_validate(slot, 0, need < first ? need : first, seq1, v);
+    uint64_t bytes = info & 0xffffffu;
+    if (!eager) bytes |= (uint64_t)lane_u32(slot_word_value(v), SF_BYTES_HI) << 24;
     uint64_t n = bytes;
     int err = MPI_SUCCESS;
     if (bytes > q->bytes) {
         n = q->bytes;
         err = MPI_ERR_TRUNCATE;
     }
-    if (flags & SLOT_EAGER) {
-        void* dst = (void*)(uintptr_t)q->buf;
-        if (is_private_ptr(dst))
-            copy_all_lanes(dst, (uint8_t*)hdr + kSlotHeaderBytes, n);
-        else
-            copy_part(dst, (uint8_t*)hdr + kSlotHeaderBytes, n, lane_id(), 64);
+    if (eager) {
+        uint8_t* dst = (uint8_t*)(uintptr_t)q->buf;
+        const bool priv = is_private_ptr(dst);
+        for (int base = 0; base < need; base += 64) {
+            if (base > 0) {
+                const int w = base + lane_id();
+                v = w < need ? ld_relaxed_agent(&slot[w]) : 0;
+                v = slot_validate(slot, base, need, seq1, v);
+            }
+            const uint32_t val = slot_word_value(v);
+            if (priv) {
+                // Every lane fills its own copy of a private buffer.
+                const int lim = need - base < 64 ? need - base : 64;
+                for (int j = 0; j < lim; j++) {
+                    const uint32_t x = lane_u32(val, j);
+                    if (base + j >= SF_PAYLOAD) slot_payload_put(dst, n, (uint64_t)(base + j - SF_PAYLOAD), x);
+                }
+            } else {
+                const int w = base + lane_id();
+                if (w >= SF_PAYLOAD && w < need) slot_payload_put(dst, n, (uint64_t)(w - SF_PAYLOAD), val);
+            }
+        }
     } else {
         // Rendezvous: only the match happens here.  The copy out of the
         // sender's buffer runs later, from a wait or test call
         // (finish_rendezvous), so that an MPI_Irecv that finds its message
         // already announced returns at once and the rank's own sends get
         // posted before it spends time copying.
-        q->rv_src = uniform_u64(hdr->src_addr);
-        q->rv_done = uniform_u64(hdr->done_addr);
-        q->rv_val = uniform_u32(hdr->done_val);
+        const uint32_t val = slot_word_value(v);
+        q->rv_src = ((uint64_t)lane_u32(val, SF_SRC_HI) << 32) | lane_u32(val, SF_SRC_LO);
+        q->rv_done = ((uint64_t)lane_u32(val, SF_DONE_HI) << 32) | lane_u32(val, SF_DONE_LO);
+        q->rv_val = lane_u32(val, SF_DONE_VAL);
     }
     // Mark consumed; publish the new head once the oldest slots are done.
+    // Every word of the slot has been read (validated above), so the sender
+    // may reuse it as soon as it sees the head: no fence is needed.
     RxChan* rc = &rxc()[src];
     const uint32_t h0 = rc->head;
     uint64_t mask = rc->mask | (1ull << (seq - h0));
@@ -280,12 +365,9 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
     }
     rc->mask = mask;
     rc->head = h;
-    if (h != h0) {
-        fence_release_agent();
-        if (elected()) st_relaxed_agent(&world()->chan_head[(uint64_t)me * L->nranks + src], h);
-    }
-    q->state = (flags & SLOT_EAGER) ? RS_COMPLETE : RS_MATCHED;
-    if (!(flags & SLOT_EAGER)) L->pending_copies++;
+    if (h != h0 && elected()) st_relaxed_agent(&world()->chan_head[(uint64_t)me * L->nranks + src], h);
+    q->state = eager ? RS_COMPLETE : RS_MATCHED;
+    if (!eager) L->pending_copies++;
     q->got = n;
     q->st_source = src;
     q->st_tag = env_tag(env);
@@ -328,7 +410,7 @@ __device__ inline bool match_from(int idx, int src, Found* f) {
         if ((rc.mask >> j) & 1ull) {
             seen = true;
         } else {
-            env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, rc.head + (uint32_t)j))->env);
+            env = ld_relaxed_agent((const uint64_t*)slot_at(L->rank, src, rc.head + (uint32_t)j));
             seen = env_seq(env) == rc.head + (uint32_t)j + 1u;
             ok = seen && env_ctx(env) == q->ctx && (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
         }
@@ -368,7 +450,7 @@ __device__ inline bool match_any(int idx, Found* f) {
             if ((rc.mask >> j) & 1ull) {
                 seen = true;
             } else {
-                env = ld_relaxed_agent(&((SlotHdr*)slot_at(L->rank, src, seq))->env);
+                env = ld_relaxed_agent((const uint64_t*)slot_at(L->rank, src, seq));
                 seen = env_seq(env) == seq + 1u;
                 ok = seen && env_ctx(env) == q->ctx && (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
             }
@@ -429,6 +511,7 @@ __device__ inline int earlier_taker(int pos, const Found& f) {
 __device__ inline void finish_rendezvous(int idx) {
     RankLds* L = ctx();
     Req* q = &reqs()[idx];
+    fence_acquire_agent();  // pairs with the sender's release before it posted
     rank_copy((void*)(uintptr_t)q->buf, (const void*)(uintptr_t)q->rv_src, q->got);
     fence_release_agent();
     if (elected()) st_relaxed_agent((uint32_t*)(uintptr_t)q->rv_done, q->rv_val);

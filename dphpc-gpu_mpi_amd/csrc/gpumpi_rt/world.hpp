@@ -18,7 +18,6 @@ namespace gpumpi {
 namespace rt {
 
 constexpr int kMaxWaves = 16;        // 1024-thread workgroup
-constexpr int kSlotHeaderBytes = 64; // channel slot header
 constexpr int kMaxChanSlots = 64;    // consumed-mask width per channel
 constexpr int kMaxCommSlots = 8;     // words of per-member collective slot
 
@@ -28,8 +27,8 @@ struct Config {
     int threads;          // threads per rank workgroup (multiple of 64)
     int waves;            // threads / 64
     int chan_slots;       // slots per (src,dst) channel, power of two <= 64
-    int slot_bytes;       // header + eager payload, multiple of 64
-    int eager_bytes;      // slot_bytes - header
+    int slot_bytes;       // slot_words_for(eager_bytes) * 8, a multiple of 64
+    int eager_bytes;      // largest message carried inside a slot
     int max_requests;     // request pool per rank (LDS)
     int helper_min_bytes; // copies at least this large use every wave
     uint64_t staging_bytes; // per-rank bounce buffer for private/LDS buffers
@@ -54,20 +53,33 @@ __host__ __device__ inline int env_tag(uint64_t e) { return (int)(int16_t)(uint1
 
 enum SlotFlags : uint32_t { SLOT_EAGER = 1u, SLOT_RNDV = 2u };
 
-// One message slot of a (src,dst) channel; the eager payload follows the
-// header inside the slot (slot_bytes - 64 bytes of it).
-struct alignas(64) SlotHdr {
-    uint64_t env;       // see pack_env
-    uint64_t bytes;     // message length
-    uint64_t src_addr;  // rendezvous: sender's buffer
-    uint64_t done_addr; // rendezvous: sender's completion word
-    uint32_t done_val;  // value the receiver stores there
-    uint32_t flags;     // SlotFlags
-    uint32_t src_world; // sender's world rank
-    uint32_t pad0;
-    uint64_t pad1[2];
-};
-static_assert(sizeof(SlotHdr) == kSlotHeaderBytes, "slot header is one 64B line");
+// A channel slot is an array of 8-byte words, each stored and loaded whole
+// (single-copy atomic) and each carrying the message's sequence number+1 in
+// its upper half: [63:32] seq+1, [31:0] value.  A receiver accepts a word
+// only when its upper half is the sequence it expects, so a slot needs no
+// fences between payload and envelope and no ordering among its words: the
+// sender writes every word at once, the receiver re-reads until all match.
+//   word 0  envelope: value = [31:16] context, [15:0] tag (pack_env)
+//   word 1  info:     value = [31:24] SlotFlags, [23:0] message bytes
+//   eager:       words 2.. payload, 4 bytes per word (little-endian)
+//   rendezvous:  words 2,3 sender buffer (lo, hi); 4,5 done-word address
+//                (lo, hi); 6 value for the done word; 7 message bytes >> 24
+enum SlotField { SF_ENV = 0, SF_INFO = 1, SF_PAYLOAD = 2, SF_SRC_LO = 2, SF_SRC_HI = 3, SF_DONE_LO = 4,
+                 SF_DONE_HI = 5, SF_DONE_VAL = 6, SF_BYTES_HI = 7, SF_RNDV_WORDS = 8 };
+constexpr int kMaxEagerBytes = (1 << 24) - 1;
+__host__ __device__ inline uint64_t slot_word(uint32_t seq1, uint32_t value) {
+    return ((uint64_t)seq1 << 32) | value;
+}
+__host__ __device__ inline uint32_t slot_word_seq(uint64_t w) { return (uint32_t)(w >> 32); }
+__host__ __device__ inline uint32_t slot_word_value(uint64_t w) { return (uint32_t)w; }
+// Words a message of `bytes` occupies.
+__host__ __device__ inline int eager_words(uint64_t bytes) { return SF_PAYLOAD + (int)((bytes + 3) / 4); }
+// Bytes of one slot for an eager limit (whole 64-byte lines).
+__host__ __device__ inline int slot_bytes_for(int eager_bytes) {
+    int words = eager_words((uint64_t)eager_bytes);
+    if (words < SF_RNDV_WORDS) words = SF_RNDV_WORDS;
+    return (words * 8 + 63) / 64 * 64;
+}
 
 // Per-member entry of a communicator's collective table.
 struct alignas(64) CollSlot {

@@ -224,7 +224,7 @@ int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
         set_err("chan_slots must be a power of two <= %d", kMaxChanSlots);
         return GPUMPI_EINVAL;
     }
-    if (c.eager_bytes < 0 || c.max_requests < 1 || c.max_requests > 4096) {
+    if (c.eager_bytes < 0 || c.eager_bytes > kMaxEagerBytes || c.max_requests < 1 || c.max_requests > 4096) {
         set_err("bad eager_bytes or max_requests");
         return GPUMPI_EINVAL;
     }
@@ -235,8 +235,8 @@ int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
     g.threads = c.threads_per_rank;
     g.waves = c.threads_per_rank / 64;
     g.chan_slots = c.chan_slots;
-    g.eager_bytes = (c.eager_bytes + 63) / 64 * 64;
-    g.slot_bytes = kSlotHeaderBytes + g.eager_bytes;
+    g.eager_bytes = c.eager_bytes;
+    g.slot_bytes = slot_bytes_for(c.eager_bytes);
     g.max_requests = c.max_requests;
     g.helper_min_bytes = c.helper_min_bytes;
     g.staging_bytes = (c.staging_bytes + 511) / 512 * 512;
