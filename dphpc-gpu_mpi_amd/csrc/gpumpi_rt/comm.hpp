@@ -50,7 +50,7 @@ __device__ inline void rank_free(void* p, uint64_t bytes) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ CommShared* comm_get(MPI_Comm c) {
     const uintptr_t v = (uintptr_t)c;
-    if (v == 1) return world()->comm_world;
+    if (v == 1) return ctx()->comm_world;
     if (v == 2) return world()->comm_self + my_rank();
     return (CommShared*)c;
 }

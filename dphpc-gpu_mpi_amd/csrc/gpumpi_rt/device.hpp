@@ -137,6 +137,12 @@ struct Cmd {
 
 struct RankLds {
     World* w;
+    // World pointers used on every message, kept here (one LDS read instead
+    // of an LDS read plus a global load).
+    uint8_t* slots;
+    uint32_t* chan_head;
+    uint32_t* done_words;
+    CommShared* comm_world;
     int rank;       // world rank
     int nranks;
     int waves;
@@ -222,7 +228,7 @@ __device__ inline void copy_msg(char* dst, const char* msg, int cap) {
 // Records the first error (first writer wins), counts every error and
 // raises the abort word so every rank's waits end (errors are fatal, as
 // under MPI_ERRORS_ARE_FATAL).  Returns `code`.
-__device__ inline int raise_error(int code, int line, const char* msg) {
+__device__ inline __attribute__((noinline)) int raise_error(int code, int line, const char* msg) {
     if (elected()) {
         ErrorRecord* e = world()->err;
         int expected = 0;

@@ -26,6 +26,10 @@ __device__ inline void rank_init(World* w) {
     if (threadIdx.x == 0) {
         RankLds* L = ctx();
         L->w = w;
+        L->slots = w->slots;
+        L->chan_head = w->chan_head;
+        L->done_words = w->done_words;
+        L->comm_world = w->comm_world;
         L->rank = rank_of_block(cfg, (int)blockIdx.x);
         L->nranks = cfg.nranks;
         L->waves = cfg.waves;

@@ -15,13 +15,22 @@ __device__ inline int msg_bytes(int count, MPI_Datatype dt, uint64_t* out) {
     return MPI_SUCCESS;
 }
 
-// Resolve (comm, rank) of a p2p call to a world rank; MPI_PROC_NULL and
-// MPI_ANY_SOURCE pass through.
-__device__ inline int p2p_peer(MPI_Comm comm, int rank, bool recv, CommShared** cout, int* wout) {
+// Resolve (comm, rank) of a p2p call to a world rank and the communicator's
+// context id; MPI_PROC_NULL and MPI_ANY_SOURCE pass through.  WORLD (context
+// 1, identity rank map) needs no lookup.
+__device__ inline int p2p_peer(MPI_Comm comm, int rank, bool recv, int* ctx_out, int* wout) {
     if (!comm_valid(comm)) return GPUMPI_FAIL(MPI_ERR_COMM, "point-to-point on MPI_COMM_NULL");
+    const bool pass = rank == MPI_PROC_NULL || (recv && rank == MPI_ANY_SOURCE);
+    if (comm == MPI_COMM_WORLD) {
+        *ctx_out = 1;
+        if (!pass && (rank < 0 || rank >= ctx()->nranks))
+            return GPUMPI_FAIL(MPI_ERR_RANK, "invalid rank in point-to-point call");
+        *wout = rank;
+        return MPI_SUCCESS;
+    }
     CommShared* c = comm_get(comm);
-    *cout = c;
-    if (rank == MPI_PROC_NULL || (recv && rank == MPI_ANY_SOURCE)) {
+    *ctx_out = c->ctx;
+    if (pass) {
         *wout = rank;
         return MPI_SUCCESS;
     }
@@ -42,10 +51,9 @@ __device__ inline int isend_common(const void* buf, int count, MPI_Datatype dt, 
     int err = msg_bytes(count, dt, &bytes);
     if (err) return err;
     if ((err = check_tag(tag, false))) return err;
-    CommShared* c;
-    int wdst;
-    if ((err = p2p_peer(comm, dest, false, &c, &wdst))) return err;
-    const int idx = start_send(buf, bytes, wdst, tag, c->ctx, comm, sync);
+    int cid, wdst;
+    if ((err = p2p_peer(comm, dest, false, &cid, &wdst))) return err;
+    const int idx = start_send(buf, bytes, wdst, tag, cid, comm, sync);
     if (idx < 0) return MPI_ERR_REQUEST;
     *idx_out = idx;
     return MPI_SUCCESS;
@@ -57,10 +65,9 @@ __device__ inline int irecv_common(void* buf, int count, MPI_Datatype dt, int so
     int err = msg_bytes(count, dt, &bytes);
     if (err) return err;
     if ((err = check_tag(tag, true))) return err;
-    CommShared* c;
-    int wsrc;
-    if ((err = p2p_peer(comm, source, true, &c, &wsrc))) return err;
-    const int idx = start_recv(buf, bytes, wsrc, tag, c->ctx, comm);
+    int cid, wsrc;
+    if ((err = p2p_peer(comm, source, true, &cid, &wsrc))) return err;
+    const int idx = start_recv(buf, bytes, wsrc, tag, cid, comm);
     if (idx < 0) return MPI_ERR_REQUEST;
     *idx_out = idx;
     return MPI_SUCCESS;
@@ -175,10 +182,18 @@ __device__ inline int MPI_Irecv(void* buf, int count, MPI_Datatype dt, int sourc
     return MPI_SUCCESS;
 }
 __device__ inline int MPI_Send(const void* buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm) {
-    int idx;
-    int err = gpumpi::rt::isend_common(buf, count, dt, dest, tag, comm, false, &idx);
+    using namespace gpumpi::rt;
+    uint64_t bytes;
+    int err = msg_bytes(count, dt, &bytes);
     if (err) return err;
-    return gpumpi::rt::wait_req(idx, MPI_STATUS_IGNORE);
+    if ((err = check_tag(tag, false))) return err;
+    int cid, wdst;
+    if ((err = p2p_peer(comm, dest, false, &cid, &wdst))) return err;
+    if (wdst == MPI_PROC_NULL) return MPI_SUCCESS;
+    if (send_direct(buf, bytes, wdst, tag, cid)) return MPI_SUCCESS;
+    const int idx = start_send(buf, bytes, wdst, tag, cid, comm, false);
+    if (idx < 0) return MPI_ERR_REQUEST;
+    return wait_req(idx, MPI_STATUS_IGNORE);
 }
 __device__ inline int MPI_Ssend(const void* buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm) {
     int idx;
@@ -188,10 +203,18 @@ __device__ inline int MPI_Ssend(const void* buf, int count, MPI_Datatype dt, int
 }
 __device__ inline int MPI_Recv(void* buf, int count, MPI_Datatype dt, int source, int tag, MPI_Comm comm,
                                MPI_Status* status) {
-    int idx;
-    int err = gpumpi::rt::irecv_common(buf, count, dt, source, tag, comm, &idx);
+    using namespace gpumpi::rt;
+    uint64_t bytes;
+    int err = msg_bytes(count, dt, &bytes);
     if (err) return err;
-    return gpumpi::rt::wait_req(idx, status);
+    if ((err = check_tag(tag, true))) return err;
+    int cid, wsrc;
+    if ((err = p2p_peer(comm, source, true, &cid, &wsrc))) return err;
+    if (wsrc >= 0 && ctx()->unmatched_any == 0 && unmatched_src()[wsrc] == 0)
+        return recv_direct(buf, bytes, wsrc, tag, cid, comm, status);
+    const int idx = start_recv(buf, bytes, wsrc, tag, cid, comm);
+    if (idx < 0) return MPI_ERR_REQUEST;
+    return wait_req(idx, status);
 }
 // A real Sendrecv (the reference's was a stub, mpi.cu:255-260): both halves
 // are posted before either is waited on, so exchanges cannot deadlock.

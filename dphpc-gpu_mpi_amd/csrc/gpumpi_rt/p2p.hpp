@@ -31,7 +31,7 @@ __device__ __forceinline__ uint8_t* slot_at(int dst, int src, uint32_t seq) {
     RankLds* L = ctx();
     const uint64_t idx = ((uint64_t)dst * (uint64_t)L->nranks + (uint64_t)src) * (uint64_t)L->Q +
                          (uint64_t)(seq & (uint32_t)(L->Q - 1));
-    return world()->slots + idx * (uint64_t)L->slot_bytes;
+    return L->slots + idx * (uint64_t)L->slot_bytes;
 }
 
 // ---------------------------------------------------------------------------
@@ -65,7 +65,7 @@ __device__ inline void ring_pop_dead() {
 }
 
 // Squeeze dead entries out of the ring, keeping posting order.
-__device__ inline void ring_compact() {
+__device__ inline __attribute__((noinline)) void ring_compact() {
     RankLds* L = ctx();
     const int R = L->R;
     int out = L->ring_head;
@@ -82,7 +82,7 @@ __device__ inline void ring_compact() {
 
 __device__ inline void progress(bool copies = true);
 
-__device__ inline int req_alloc() {
+__device__ inline __attribute__((noinline)) int req_alloc() {
     RankLds* L = ctx();
     const int R = L->R;
     const int nwords = (R + 63) / 64;
@@ -134,7 +134,7 @@ __device__ __forceinline__ uint8_t* staging_base() {
 }
 __device__ __forceinline__ uint64_t staging_half() { return world()->cfg.staging_bytes / 2; }
 
-__device__ inline uint8_t* bounce_alloc(uint64_t bytes) {
+__device__ inline __attribute__((noinline)) uint8_t* bounce_alloc(uint64_t bytes) {
     RankLds* L = ctx();
     const uint64_t need = (bytes + 255) & ~(uint64_t)255;
     if (L->bounce_users == 0) L->bounce_top = 0;
@@ -170,67 +170,88 @@ __device__ __forceinline__ void slot_payload_put(uint8_t* p, uint64_t n, uint64_
 }
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, int l) { return (uint32_t)__shfl((int)v, l); }
 
-// Post send request idx into its channel if a slot is free.  Every word of
-// the slot (envelope, info, payload or rendezvous fields) is one 8-byte
-// store tagged with seq+1, written by the lanes in parallel with no fence:
-// the receiver validates each word by its tag.  A rendezvous send first
-// releases its buffer, which the receiver will read directly.
-__device__ inline bool try_post_send(int idx) {
+// Sequence number of the next free slot of channel me->dst, or false when
+// all Q slots are still unconsumed.
+__device__ __forceinline__ bool slot_reserve(int dst, uint32_t* seq) {
     RankLds* L = ctx();
-    Req* q = &reqs()[idx];
-    const int dst = q->peer, me = L->rank;
     TxChan* tc = &txc()[dst];
     const uint32_t s = tc->seq;
     const uint32_t Q = (uint32_t)L->Q;
     if (s - tc->head_cache >= Q) {
         // The receiver publishes its head only after it has read the slots
         // it consumed, so no fence is needed before reusing them.
-        const uint32_t h = uniform_u32(ld_relaxed_agent(&world()->chan_head[(uint64_t)dst * L->nranks + me]));
+        const uint32_t h = uniform_u32(ld_relaxed_agent(&L->chan_head[(uint64_t)dst * L->nranks + L->rank]));
         tc->head_cache = h;
         if (s - h >= Q) return false;
     }
-    uint64_t* slot = (uint64_t*)slot_at(dst, me, s);
-    const uint32_t seq1 = s + 1;
+    *seq = s;
+    return true;
+}
+
+// Write an eager message into slot `seq` of channel me->dst.  Every word of
+// the slot (envelope, info, payload) is one 8-byte store tagged with seq+1,
+// written by the lanes in parallel with no fence: the receiver validates
+// each word by its tag.
+__device__ __forceinline__ void slot_write_eager(int dst, uint32_t seq, const void* buf, uint64_t n, int ctxid,
+                                                 int tag) {
+    RankLds* L = ctx();
+    uint64_t* slot = (uint64_t*)slot_at(dst, L->rank, seq);
+    const uint32_t seq1 = seq + 1;
+    const uint8_t* src = (const uint8_t*)buf;
+    const int nw = eager_words(n);
+    for (int w = lane_id(); w < nw; w += 64) {
+        uint32_t v;
+        if (w == SF_ENV) v = slot_word_value(pack_env(0, ctxid, tag));
+        else if (w == SF_INFO) v = (SLOT_EAGER << 24) | (uint32_t)n;
+        else v = slot_payload_get(src, n, (uint64_t)(w - SF_PAYLOAD));
+        st_relaxed_agent(&slot[w], slot_word(seq1, v));
+    }
+    txc()[dst].seq = seq + 1;
+}
+
+// Post send request idx into its channel if a slot is free.  A rendezvous
+// send first releases its buffer, which the receiver will read directly,
+// then announces address, length and its completion word.
+__device__ inline bool try_post_send(int idx) {
+    RankLds* L = ctx();
+    Req* q = &reqs()[idx];
+    const int dst = q->peer, me = L->rank;
+    uint32_t s;
+    if (!slot_reserve(dst, &s)) return false;
     const uint64_t n = q->bytes;
     const bool eager = n <= (uint64_t)L->eager_bytes && !(q->flags & RF_SYNC);
-    const uint32_t envv = slot_word_value(pack_env(0, q->ctx, q->tag));
-    const int lane = lane_id();
     if (eager) {
-        const uint8_t* src = (const uint8_t*)(uintptr_t)q->buf;
-        const int nw = eager_words(n);
-        for (int w = lane; w < nw; w += 64) {
-            uint32_t v;
-            if (w == SF_ENV) v = envv;
-            else if (w == SF_INFO) v = (SLOT_EAGER << 24) | (uint32_t)n;
-            else v = slot_payload_get(src, n, (uint64_t)(w - SF_PAYLOAD));
-            st_relaxed_agent(&slot[w], slot_word(seq1, v));
-        }
-    } else {
-        fence_release_agent();
-        const uint64_t done = (uint64_t)(uintptr_t)&world()->done_words[(uint64_t)me * L->R + idx];
-        if (lane < SF_RNDV_WORDS) {
-            uint32_t v = 0;
-            switch (lane) {
-            case SF_ENV: v = envv; break;
-            case SF_INFO: v = (SLOT_RNDV << 24) | (uint32_t)(n & 0xffffffu); break;
-            case SF_SRC_LO: v = (uint32_t)q->buf; break;
-            case SF_SRC_HI: v = (uint32_t)(q->buf >> 32); break;
-            case SF_DONE_LO: v = (uint32_t)done; break;
-            case SF_DONE_HI: v = (uint32_t)(done >> 32); break;
-            case SF_DONE_VAL: v = q->stamp; break;
-            default: v = (uint32_t)(n >> 24); break;
-            }
-            st_relaxed_agent(&slot[lane], slot_word(seq1, v));
-        }
+        slot_write_eager(dst, s, (const void*)(uintptr_t)q->buf, n, q->ctx, q->tag);
+        q->state = RS_COMPLETE;
+        return true;
     }
-    tc->seq = s + 1;
-    q->state = eager ? RS_COMPLETE : RS_WAIT_DONE;
+    uint64_t* slot = (uint64_t*)slot_at(dst, me, s);
+    const uint32_t seq1 = s + 1;
+    const int lane = lane_id();
+    fence_release_agent();
+    const uint64_t done = (uint64_t)(uintptr_t)&L->done_words[(uint64_t)me * L->R + idx];
+    if (lane < SF_RNDV_WORDS) {
+        uint32_t v = 0;
+        switch (lane) {
+        case SF_ENV: v = slot_word_value(pack_env(0, q->ctx, q->tag)); break;
+        case SF_INFO: v = (SLOT_RNDV << 24) | (uint32_t)(n & 0xffffffu); break;
+        case SF_SRC_LO: v = (uint32_t)q->buf; break;
+        case SF_SRC_HI: v = (uint32_t)(q->buf >> 32); break;
+        case SF_DONE_LO: v = (uint32_t)done; break;
+        case SF_DONE_HI: v = (uint32_t)(done >> 32); break;
+        case SF_DONE_VAL: v = q->stamp; break;
+        default: v = (uint32_t)(n >> 24); break;
+        }
+        st_relaxed_agent(&slot[lane], slot_word(seq1, v));
+    }
+    txc()[dst].seq = s + 1;
+    q->state = RS_WAIT_DONE;
     return true;
 }
 
 // Start a send of `bytes` at `buf` to world rank `dst`.  Returns the request
 // index, or -1 after an error.
-__device__ inline int start_send(const void* buf, uint64_t bytes, int dst, int tag, int ctxid,
+__device__ inline __attribute__((noinline)) int start_send(const void* buf, uint64_t bytes, int dst, int tag, int ctxid,
                                  MPI_Comm comm, bool sync) {
     RankLds* L = ctx();
     const int idx = req_alloc();
@@ -293,10 +314,24 @@ __device__ inline uint64_t slot_validate(const uint64_t* slot, int base, int nee
     }
 }
 
-// Deliver message `seq` of channel src->me to receive request `idx`.
-__device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
+// What reading a slot yields: bytes delivered (eager) or announced
+// (rendezvous, with the sender's buffer and completion word), and the error.
+struct Taken {
+    uint64_t got;
+    int err;
+    bool rndv;
+    uint64_t rv_src, rv_done;
+    uint32_t rv_val;
+};
+
+// Read message `seq` of channel src->me: copy an eager payload into `dstv`
+// (capacity `cap` bytes; a larger message is truncated with
+// MPI_ERR_TRUNCATE), or return the rendezvous announcement; then mark the
+// slot consumed.  `pre`, when has_pre, is lane l's word l of the slot,
+// loaded by the matcher in the same round trip as the envelope.
+__device__ inline void slot_take(int src, uint32_t seq, void* dstv, uint64_t cap, bool has_pre, uint64_t pre,
+                                 Taken* t) {
     RankLds* L = ctx();
-    Req* q = &reqs()[idx];
     const int me = L->rank;
     const uint64_t* slot = (const uint64_t*)slot_at(me, src, seq);
     const uint32_t seq1 = seq + 1;
@@ -304,23 +339,24 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
     // Words 0..63 (at most the slot) in one round trip; the info word then
     // says how many of them (and of later chunks) the message uses.
     const int first = swords < 64 ? swords : 64;
-    uint64_t v = lane_id() < first ? ld_relaxed_agent(&slot[lane_id()]) : 0;
+    uint64_t v = has_pre ? pre : (lane_id() < first ? ld_relaxed_agent(&slot[lane_id()]) : 0);
     v = slot_validate(slot, 0, SF_INFO + 1, seq1, v);
     const uint32_t info = lane_u32(slot_word_value(v), SF_INFO);
-    const uint32_t flags = info >> 24;
-    const bool eager = (flags & SLOT_EAGER) != 0;
+    const bool eager = ((info >> 24) & SLOT_EAGER) != 0;
     const int need = eager ? eager_words(info & 0xffffffu) : (int)SF_RNDV_WORDS;
     v = slot_validate(slot, 0, need < first ? need : first, seq1, v);
     uint64_t bytes = info & 0xffffffu;
     if (!eager) bytes |= (uint64_t)lane_u32(slot_word_value(v), SF_BYTES_HI) << 24;
-    uint64_t n = bytes;
-    int err = MPI_SUCCESS;
-    if (bytes > q->bytes) {
-        n = q->bytes;
-        err = MPI_ERR_TRUNCATE;
+    t->got = bytes;
+    t->err = MPI_SUCCESS;
+    if (bytes > cap) {
+        t->got = cap;
+        t->err = MPI_ERR_TRUNCATE;
     }
+    t->rndv = !eager;
     if (eager) {
-        uint8_t* dst = (uint8_t*)(uintptr_t)q->buf;
+        uint8_t* dst = (uint8_t*)dstv;
+        const uint64_t n = t->got;
         const bool priv = is_private_ptr(dst);
         for (int base = 0; base < need; base += 64) {
             if (base > 0) {
@@ -342,15 +378,10 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
             }
         }
     } else {
-        // Rendezvous: only the match happens here.  The copy out of the
-        // sender's buffer runs later, from a wait or test call
-        // (finish_rendezvous), so that an MPI_Irecv that finds its message
-        // already announced returns at once and the rank's own sends get
-        // posted before it spends time copying.
         const uint32_t val = slot_word_value(v);
-        q->rv_src = ((uint64_t)lane_u32(val, SF_SRC_HI) << 32) | lane_u32(val, SF_SRC_LO);
-        q->rv_done = ((uint64_t)lane_u32(val, SF_DONE_HI) << 32) | lane_u32(val, SF_DONE_LO);
-        q->rv_val = lane_u32(val, SF_DONE_VAL);
+        t->rv_src = ((uint64_t)lane_u32(val, SF_SRC_HI) << 32) | lane_u32(val, SF_SRC_LO);
+        t->rv_done = ((uint64_t)lane_u32(val, SF_DONE_HI) << 32) | lane_u32(val, SF_DONE_LO);
+        t->rv_val = lane_u32(val, SF_DONE_VAL);
     }
     // Mark consumed; publish the new head once the oldest slots are done.
     // Every word of the slot has been read (validated above), so the sender
@@ -365,16 +396,43 @@ __device__ inline void deliver(int idx, int src, uint32_t seq, uint64_t env) {
     }
     rc->mask = mask;
     rc->head = h;
-    if (h != h0 && elected()) st_relaxed_agent(&world()->chan_head[(uint64_t)me * L->nranks + src], h);
-    q->state = eager ? RS_COMPLETE : RS_MATCHED;
-    if (!eager) L->pending_copies++;
-    q->got = n;
+    if (h != h0 && elected()) st_relaxed_agent(&L->chan_head[(uint64_t)me * L->nranks + src], h);
+}
+
+// Copy a rendezvous message out of the sender's buffer (every wave of the
+// rank), then release the sender's completion word.
+__device__ inline __attribute__((noinline)) void rendezvous_copy(void* dst, const Taken& t) {
+    fence_acquire_agent();  // pairs with the sender's release before it posted
+    rank_copy(dst, (const void*)(uintptr_t)t.rv_src, t.got);
+    fence_release_agent();
+    if (elected()) st_relaxed_agent((uint32_t*)(uintptr_t)t.rv_done, t.rv_val);
+}
+
+// Deliver message `seq` of channel src->me to receive request `idx`.  A
+// rendezvous message is only matched here: the copy runs later, from a wait
+// or test call (finish_rendezvous), so that an MPI_Irecv that finds its
+// message already announced returns at once and the rank's own sends get
+// posted before it spends time copying.
+__device__ inline __attribute__((noinline)) void deliver(int idx, int src, uint32_t seq, uint64_t env, bool has_pre = false,
+                               uint64_t pre = 0) {
+    RankLds* L = ctx();
+    Req* q = &reqs()[idx];
+    Taken t;
+    slot_take(src, seq, (void*)(uintptr_t)q->buf, q->bytes, has_pre, pre, &t);
+    if (t.rndv) {
+        q->rv_src = t.rv_src;
+        q->rv_done = t.rv_done;
+        q->rv_val = t.rv_val;
+        L->pending_copies++;
+    }
+    q->state = t.rndv ? RS_MATCHED : RS_COMPLETE;
+    q->got = t.got;
     q->st_source = src;
     q->st_tag = env_tag(env);
-    q->st_error = err;
+    q->st_error = t.err;
     if (q->peer < 0) L->unmatched_any--;
     else unmatched_src()[q->peer] = (uint16_t)(unmatched_src()[q->peer] - 1);
-    if (err != MPI_SUCCESS) GPUMPI_FAIL(err, "message truncated: receive buffer too small");
+    if (t.err != MPI_SUCCESS) GPUMPI_FAIL(t.err, "message truncated: receive buffer too small");
 }
 
 // Slots of one channel are read by different lanes with independent loads,
@@ -395,25 +453,35 @@ struct Found {
     int src;
     uint32_t seq;
     uint64_t env;
+    bool has_pre;  // pre holds this lane's word of the matched slot's first chunk
+    uint64_t pre;
 };
 
-// Look for the oldest message of channel src->me matching request idx.
-__device__ inline bool match_from(int idx, int src, Found* f) {
+// Look for the oldest message of channel src->me with context ctxid and tag
+// `tag` (or any tag).
+__device__ inline bool scan_channel(int src, int ctxid, int tag, Found* f) {
     RankLds* L = ctx();
-    const Req* q = &reqs()[idx];
     const int Q = L->Q;
     const RxChan rc = rxc()[src];
     const int j = lane_id();
     bool ok = false, seen = false;
     uint64_t env = 0;
+    // Alongside the envelopes, the first chunk of the oldest slot: when that
+    // slot is the match (the common, in-order case) delivery needs no
+    // further round trip.
+    const uint64_t* head_slot = (const uint64_t*)slot_at(L->rank, src, rc.head);
+    const int hw = L->slot_bytes / 8 < 64 ? L->slot_bytes / 8 : 64;
     if (j < Q) {
         if ((rc.mask >> j) & 1ull) {
             seen = true;
         } else {
             env = ld_relaxed_agent((const uint64_t*)slot_at(L->rank, src, rc.head + (uint32_t)j));
-            seen = env_seq(env) == rc.head + (uint32_t)j + 1u;
-            ok = seen && env_ctx(env) == q->ctx && (q->tag == MPI_ANY_TAG || env_tag(env) == q->tag);
         }
+    }
+    const uint64_t hv = j < hw ? ld_relaxed_agent(&head_slot[j]) : 0;
+    if (j < Q && !seen) {
+        seen = env_seq(env) == rc.head + (uint32_t)j + 1u;
+        ok = seen && env_ctx(env) == ctxid && (tag == MPI_ANY_TAG || env_tag(env) == tag);
     }
     const uint64_t b = ballot(ok) & arrived_prefix(ballot(seen), Q);
     if (!b) return false;
@@ -421,7 +489,15 @@ __device__ inline bool match_from(int idx, int src, Found* f) {
     f->src = src;
     f->seq = rc.head + (uint32_t)first;
     f->env = uniform_u64(__shfl(env, first));
+    f->has_pre = first == 0;
+    f->pre = hv;
     return true;
+}
+
+// Look for the oldest message of channel src->me matching request idx.
+__device__ __forceinline__ bool match_from(int idx, int src, Found* f) {
+    const Req* q = &reqs()[idx];
+    return scan_channel(src, q->ctx, q->tag, f);
 }
 
 // MPI_ANY_SOURCE: scan the channels of every member of the communicator,
@@ -464,6 +540,8 @@ __device__ inline bool match_any(int idx, Found* f) {
             f->env = uniform_u64(__shfl(env, first));
             f->src = uniform_i32(__shfl(src, first));
             f->seq = uniform_u32(__shfl((int)seq, first));
+            f->has_pre = false;
+            f->pre = 0;
             return true;
         }
     }
@@ -480,7 +558,7 @@ __device__ __forceinline__ bool find_match(int idx, Found* f) {
 __device__ __forceinline__ bool try_match(int idx) {
     Found f;
     if (!find_match(idx, &f)) return false;
-    deliver(idx, f.src, f.seq, f.env);
+    deliver(idx, f.src, f.seq, f.env, f.has_pre, f.pre);
     return true;
 }
 
@@ -508,13 +586,15 @@ __device__ inline int earlier_taker(int pos, const Found& f) {
 
 // Copy a matched rendezvous message out of the sender's buffer (every wave
 // of the rank), then release the sender's done word.
-__device__ inline void finish_rendezvous(int idx) {
+__device__ inline __attribute__((noinline)) void finish_rendezvous(int idx) {
     RankLds* L = ctx();
     Req* q = &reqs()[idx];
-    fence_acquire_agent();  // pairs with the sender's release before it posted
-    rank_copy((void*)(uintptr_t)q->buf, (const void*)(uintptr_t)q->rv_src, q->got);
-    fence_release_agent();
-    if (elected()) st_relaxed_agent((uint32_t*)(uintptr_t)q->rv_done, q->rv_val);
+    Taken t;
+    t.got = q->got;
+    t.rv_src = q->rv_src;
+    t.rv_done = q->rv_done;
+    t.rv_val = q->rv_val;
+    rendezvous_copy((void*)(uintptr_t)q->buf, t);
     q->state = RS_COMPLETE;
     L->pending_copies--;
 }
@@ -540,7 +620,7 @@ __device__ inline void progress(bool copies) {
             }
             if (q->state == RS_WAIT_DONE) {
                 const uint32_t v = uniform_u32(
-                    ld_relaxed_agent(&world()->done_words[(uint64_t)L->rank * R + idx]));
+                    ld_relaxed_agent(&L->done_words[(uint64_t)L->rank * R + idx]));
                 if (v == q->stamp) {
                     fence_acquire_agent();
                     q->state = RS_COMPLETE;
@@ -551,7 +631,7 @@ __device__ inline void progress(bool copies) {
             Found f;
             if (!find_match(idx, &f)) continue;
             const int e = earlier_taker(i, f);
-            if (e < 0) deliver(idx, f.src, f.seq, f.env);
+            if (e < 0) deliver(idx, f.src, f.seq, f.env, f.has_pre, f.pre);
             else try_match(e);  // takes its own oldest match: f or an older one
         }
     }
@@ -566,7 +646,7 @@ __device__ inline void progress(bool copies) {
 
 // Start a receive of up to `bytes` into `buf` from world rank `src`
 // (or -1 = any member of comm).
-__device__ inline int start_recv(void* buf, uint64_t bytes, int src, int tag, int ctxid, MPI_Comm comm) {
+__device__ inline __attribute__((noinline)) int start_recv(void* buf, uint64_t bytes, int src, int tag, int ctxid, MPI_Comm comm) {
     RankLds* L = ctx();
     const int idx = req_alloc();
     if (idx < 0) return -1;
@@ -604,7 +684,8 @@ __device__ inline void fill_status(int idx, MPI_Status* st) {
     if (!st) return;
     const Req* q = &reqs()[idx];
     int src = q->st_source;
-    if (q->kind == REQ_RECV && src >= 0) src = comm_rank_of_world(comm_get((MPI_Comm)(uintptr_t)q->comm), src);
+    if (q->kind == REQ_RECV && src >= 0 && (MPI_Comm)(uintptr_t)q->comm != MPI_COMM_WORLD)
+        src = comm_rank_of_world(comm_get((MPI_Comm)(uintptr_t)q->comm), src);
     st->MPI_SOURCE = q->kind == REQ_RECV ? src : MPI_ANY_SOURCE;
     st->MPI_TAG = q->kind == REQ_RECV ? q->st_tag : MPI_ANY_TAG;
     st->MPI_ERROR = q->st_error;
@@ -613,9 +694,26 @@ __device__ inline void fill_status(int idx, MPI_Status* st) {
 }
 
 // Wait for request idx; fills the status and frees the request.
-__device__ inline int wait_req(int idx, MPI_Status* st) {
+__device__ inline __attribute__((noinline)) int wait_req(int idx, MPI_Status* st) {
+    RankLds* L = ctx();
     Req* q = &reqs()[idx];
     Spinner sp;
+    // A receive that is the only unmatched one able to take messages from its
+    // source polls that channel alone; the full progress pass (other requests
+    // of this rank) runs every 16th poll.
+    if (q->kind == REQ_RECV && q->state == RS_PENDING && q->peer >= 0 && L->unmatched_any == 0 &&
+        unmatched_src()[q->peer] == 1) {
+        int k = 0;
+        while (q->state == RS_PENDING) {
+            Found f;
+            if (match_from(idx, q->peer, &f)) {
+                deliver(idx, f.src, f.seq, f.env, f.has_pre, f.pre);
+                break;
+            }
+            if ((++k & 15) == 0) progress();
+            if (!sp.keep_going()) break;
+        }
+    }
     while (q->state != RS_COMPLETE) {
         progress();
         if (q->state == RS_COMPLETE) break;
@@ -628,6 +726,48 @@ __device__ inline int wait_req(int idx, MPI_Status* st) {
     const int err = q->st_error;
     req_release(idx);
     return err;
+}
+
+// Blocking eager send without a request: legal when no earlier send of this
+// rank is waiting for a slot (the channel order is then the program order).
+// Returns false when the message must take the general path.
+__device__ inline bool send_direct(const void* buf, uint64_t n, int dst, int tag, int ctxid) {
+    RankLds* L = ctx();
+    if (L->pending_sends != 0 || n > (uint64_t)L->eager_bytes) return false;
+    uint32_t s;
+    if (!slot_reserve(dst, &s)) return false;
+    slot_write_eager(dst, s, buf, n, ctxid, tag);
+    return true;
+}
+
+// Blocking receive without a request: legal when no posted receive of this
+// rank could take a message from `src` (no unmatched receive names `src` or
+// MPI_ANY_SOURCE), so the oldest matching message of that channel is this
+// receive's.  Polls the one channel; the full progress pass (the rank's
+// other requests) runs every 16th poll.
+__device__ inline int recv_direct(void* buf, uint64_t cap, int src, int tag, int ctxid, MPI_Comm comm,
+                                  MPI_Status* st) {
+    Spinner sp;
+    int k = 0;
+    for (;;) {
+        Found f;
+        if (scan_channel(src, ctxid, tag, &f)) {
+            Taken t;
+            slot_take(f.src, f.seq, buf, cap, f.has_pre, f.pre, &t);
+            if (t.rndv) rendezvous_copy(buf, t);
+            if (st) {
+                st->MPI_SOURCE = comm == MPI_COMM_WORLD ? src : comm_rank_of_world(comm_get(comm), src);
+                st->MPI_TAG = env_tag(f.env);
+                st->MPI_ERROR = t.err;
+                st->gpumpi_count_lo = (int)(uint32_t)t.got;
+                st->gpumpi_count_hi = (int)(uint32_t)(t.got >> 32);
+            }
+            if (t.err != MPI_SUCCESS) GPUMPI_FAIL(t.err, "message truncated: receive buffer too small");
+            return t.err;
+        }
+        if ((++k & 15) == 0) progress();
+        if (!sp.keep_going()) return MPI_ERR_OTHER;
+    }
 }
 
 }  // namespace rt
