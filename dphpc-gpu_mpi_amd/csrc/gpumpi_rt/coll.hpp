@@ -147,11 +147,11 @@ __device__ inline void run_cmd(const Cmd& c, int t, int nt) {
         copy_part((void*)(uintptr_t)c.a[0], (const void*)(uintptr_t)c.a[1], c.a[2], t, nt);
         break;
     case CMD_REDUCE: {
-        const CollSlot* tab = (const CollSlot*)(uintptr_t)c.a[0];
+        const uint64_t* mem = (const uint64_t*)(uintptr_t)c.a[0];  // members_lds()
         FoldArgs A;
-        A.ins_w = &tab[0].w[SW_SBUF];
-        A.outs_w = &tab[0].w[SW_RBUF];
-        A.stride_words = kMaxCommSlots;
+        A.ins_w = &mem[0];
+        A.outs_w = &mem[1];
+        A.stride_words = 2;
         A.nmembers = c.i2;
         A.root = c.i3;
         A.vec = c.i4;
@@ -203,6 +203,16 @@ __device__ inline void publish(CollCtx& cc, int word, uint64_t v) {
     cc.tab[cc.me].w[word] = v;
 }
 __device__ inline uint64_t peer_word(CollCtx& cc, int r, int word) { return uniform_u64(cc.tab[r].w[word]); }
+
+// Copy every member's `word_in` and `word_out` table entries into
+// members_lds() (wave 0; the helpers see it through the command handoff).
+__device__ inline void load_members(CollCtx& cc, int word_in, int word_out) {
+    uint64_t* m = members_lds();
+    for (int r = lane_id(); r < cc.size; r += 64) {
+        m[2 * r] = cc.tab[r].w[word_in];
+        m[2 * r + 1] = cc.tab[r].w[word_out];
+    }
+}
 
 // Are the given words of every member 16-byte aligned?
 __device__ inline bool all_aligned16(CollCtx& cc, int word_a, int word_b, int only_root) {
@@ -258,7 +268,8 @@ __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int coun
         c.i2 = cc.size;
         c.i3 = all ? -1 : root;
         c.i4 = all_aligned16(cc, SW_SBUF, SW_RBUF, all ? -1 : root) ? 1 : 0;
-        c.a[0] = (uint64_t)(uintptr_t)cc.tab;
+        load_members(cc, SW_SBUF, SW_RBUF);
+        c.a[0] = (uint64_t)(uintptr_t)members_lds();
         c.a[1] = lo;
         c.a[2] = hi;
         const uint64_t moved = (hi - lo) * (uint64_t)es * (uint64_t)(all ? 2 * cc.size : cc.size + 1);

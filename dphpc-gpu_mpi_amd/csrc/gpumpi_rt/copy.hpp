@@ -22,20 +22,64 @@ __device__ __forceinline__ bool is_global_ptr(const void* p) {
 
 constexpr int kCopyUnroll = 4;
 
+// Bytes [4*Q + r, 4*Q + r + 16) of the 32-byte concatenation lo:hi.
+template <int Q>
+__device__ __forceinline__ uint4 realign16(const uint4& lo, const uint4& hi, uint32_t r) {
+    const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    uint4 o;
+    o.x = __builtin_amdgcn_alignbyte(w[Q + 1], w[Q], r);
+    o.y = __builtin_amdgcn_alignbyte(w[Q + 2], w[Q + 1], r);
+    o.z = __builtin_amdgcn_alignbyte(w[Q + 3], w[Q + 2], r);
+    o.w = __builtin_amdgcn_alignbyte(w[Q + 4], w[Q + 3], r);
+    return o;
+}
+
+// d4[i] = source bytes [16 i + 4 Q + r, +16) for i < nvec, reading only the
+// 16-byte aligned source blocks a4[i] and a4[i + 1]: every block read holds
+// at least one byte of the source range, so no read leaves its page.
+template <int Q>
+__device__ inline void copy_realigned(uint4* d4, const uint4* a4, uint64_t nvec, uint32_t r, int t, int nt) {
+    uint64_t i = (uint64_t)t;
+    const uint64_t st = (uint64_t)nt;
+    for (; i + (kCopyUnroll - 1) * st < nvec; i += kCopyUnroll * st) {
+        uint4 lo[kCopyUnroll], hi[kCopyUnroll];
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; u++) {
+            lo[u] = a4[i + u * st];
+            hi[u] = a4[i + u * st + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyUnroll; u++) d4[i + u * st] = realign16<Q>(lo[u], hi[u], r);
+    }
+    for (; i < nvec; i += st) d4[i] = realign16<Q>(a4[i], a4[i + 1], r);
+}
+
 // Worker t of nt copies its share of [src, src+n) to dst.  Every worker
-// passes the same arguments.  Buffers may overlap only if dst == src.
+// passes the same arguments; nt >= 16.  Buffers may overlap only if
+// dst == src.  The destination is aligned to 16 bytes by a short head, then
+// moved in 16-byte vectors (dwordx4 loads and stores, several in flight per
+// lane); a source at another alignment is read as aligned vectors and
+// shifted into place.
 __device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t, int nt) {
     uint8_t* dst = (uint8_t*)dstv;
     const uint8_t* src = (const uint8_t*)srcv;
     if (n == 0 || dst == src) return;
-    const uintptr_t d = (uintptr_t)dst, s = (uintptr_t)src;
-    if (((d ^ s) & 15) == 0) {
-        uint64_t head = (16 - (d & 15)) & 15;
-        if (head > n) head = n;
-        if ((uint64_t)t < head) dst[t] = src[t];
-        const uint64_t nvec = (n - head) >> 4;
-        const uint4* s4 = (const uint4*)(src + head);
-        uint4* d4 = (uint4*)(dst + head);
+    uint64_t head = (16 - ((uintptr_t)dst & 15)) & 15;
+    if (head > n) head = n;
+    if ((uint64_t)t < head) dst[t] = src[t];
+    const uint64_t nvec = (n - head) >> 4;
+    uint4* d4 = (uint4*)(dst + head);
+    const uint8_t* s2 = src + head;
+    const uint32_t m = (uint32_t)((uintptr_t)s2 & 15);
+    if (m != 0 && !is_global_ptr(src)) {
+        // Private or LDS source at another alignment: plain bytes (whole
+        // aligned blocks are read only from global memory, where a block
+        // never leaves its page).
+        for (uint64_t i = head + (uint64_t)t; i < n; i += (uint64_t)nt) dst[i] = src[i];
+        return;
+    }
+    if (m == 0) {
+        const uint4* s4 = (const uint4*)s2;
         uint64_t i = (uint64_t)t;
         const uint64_t st = (uint64_t)nt;
         for (; i + (kCopyUnroll - 1) * st < nvec; i += kCopyUnroll * st) {
@@ -46,24 +90,17 @@ __device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t
             for (int u = 0; u < kCopyUnroll; u++) d4[i + u * st] = v[u];
         }
         for (; i < nvec; i += st) d4[i] = s4[i];
-        const uint64_t done = head + (nvec << 4);
-        const uint64_t tail = n - done;
-        if ((uint64_t)t < tail) dst[done + t] = src[done + t];
-        return;
+    } else {
+        const uint4* a4 = (const uint4*)(s2 - m);
+        switch (m >> 2) {
+        case 0: copy_realigned<0>(d4, a4, nvec, m & 3, t, nt); break;
+        case 1: copy_realigned<1>(d4, a4, nvec, m & 3, t, nt); break;
+        case 2: copy_realigned<2>(d4, a4, nvec, m & 3, t, nt); break;
+        default: copy_realigned<3>(d4, a4, nvec, m & 3, t, nt); break;
+        }
     }
-    if (((d ^ s) & 3) == 0) {
-        uint64_t head = (4 - (d & 3)) & 3;
-        if (head > n) head = n;
-        if ((uint64_t)t < head) dst[t] = src[t];
-        const uint64_t nw = (n - head) >> 2;
-        const uint32_t* s1 = (const uint32_t*)(src + head);
-        uint32_t* d1 = (uint32_t*)(dst + head);
-        for (uint64_t i = (uint64_t)t; i < nw; i += (uint64_t)nt) d1[i] = s1[i];
-        const uint64_t done = head + (nw << 2);
-        if ((uint64_t)t < n - done) dst[done + t] = src[done + t];
-        return;
-    }
-    for (uint64_t i = (uint64_t)t; i < n; i += (uint64_t)nt) dst[i] = src[i];
+    const uint64_t done = head + (nvec << 4);
+    if ((uint64_t)t < n - done) dst[done + t] = src[done + t];
 }
 
 // Every lane copies all n bytes: used when dst is the caller's private

@@ -170,7 +170,7 @@ struct RankLds {
     uint64_t bounce_top; // staging bump offset for private rendezvous sends
     int bounce_users;
     // offsets of the variable arrays from the LDS base
-    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user;
+    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_members, off_user;
     uint32_t lds_bytes;
 };
 
@@ -187,11 +187,17 @@ __device__ __forceinline__ uint16_t* unmatched_src() {
 __device__ __forceinline__ uint64_t* free_bits() {
     return (uint64_t*)(gpumpi_lds_base + ctx()->off_freebits);
 }
+// Per-collective copy of the members' two buffer addresses (input, output),
+// [2 * member + k]: read once from the communicator's table, then by every
+// wave of the rank from LDS.
+__device__ __forceinline__ uint64_t* members_lds() {
+    return (uint64_t*)(gpumpi_lds_base + ctx()->off_members);
+}
 
 // Byte size of the LDS image for a configuration (host and device agree).
 __host__ __device__ inline uint32_t align_up(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
 struct LdsLayout {
-    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user, total;
+    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_members, off_user, total;
 };
 __host__ __device__ inline LdsLayout lds_layout(int nranks, int R, int user_bytes) {
     LdsLayout L;
@@ -202,6 +208,7 @@ __host__ __device__ inline LdsLayout lds_layout(int nranks, int R, int user_byte
     L.off_tx = o; o = align_up(o + (uint32_t)(sizeof(TxChan) * nranks), 16);
     L.off_unmatched = o; o = align_up(o + (uint32_t)(2 * nranks), 16);
     L.off_freebits = o; o = align_up(o + (uint32_t)(8 * ((R + 63) / 64)), 16);
+    L.off_members = o; o = align_up(o + (uint32_t)(16 * nranks), 16);
     L.off_user = o; o = align_up(o + (uint32_t)user_bytes, 16);
     L.total = o;
     return L;

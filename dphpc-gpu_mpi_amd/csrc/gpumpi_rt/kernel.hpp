@@ -46,6 +46,7 @@ __device__ inline void rank_init(World* w) {
         L->off_tx = lay.off_tx;
         L->off_unmatched = lay.off_unmatched;
         L->off_freebits = lay.off_freebits;
+        L->off_members = lay.off_members;
         L->off_user = lay.off_user;
         L->lds_bytes = lay.total;
     }

@@ -120,55 +120,80 @@ struct FoldArgs {
     uint64_t lo, hi;
 };
 
+// Members are folded in batches of kFoldBatch: the batch's loads are all
+// issued before its values are folded in rank order, so a lane keeps
+// kFoldBatch * U vectors in flight instead of one.
+constexpr int kFoldBatch = 8;
+
 template <typename T, typename F>
 __device__ inline void fold_slice(const FoldArgs& A, int t, int nt) {
     F f;
     const int P = A.nmembers;
     constexpr int V = sizeof(T) >= 16 ? 1 : (int)(16 / sizeof(T));
     constexpr int U = 2;
+    constexpr int B = kFoldBatch;
     uint64_t lo = A.lo;
     const uint64_t hi = A.hi;
+    const int r0 = A.root >= 0 ? A.root : 0;
+    const int r1 = A.root >= 0 ? A.root + 1 : P;
     if (A.vec && V > 1) {
         struct alignas(16) Vt { T v[V]; };
         const uint64_t ngroups = (hi - lo) / V;
+        const uint64_t gbase = lo / V;
         for (uint64_t g0 = (uint64_t)t; g0 < ngroups; g0 += (uint64_t)nt * U) {
             Vt acc[U];
             bool valid[U];
+            {
+                const Vt* p = (const Vt*)(uintptr_t)A.ins_w[0];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint64_t g = g0 + (uint64_t)u * nt;
-                valid[u] = g < ngroups;
-                if (valid[u]) acc[u] = ((const Vt*)(uintptr_t)A.ins_w[0])[(lo / V) + g];
+                for (int u = 0; u < U; u++) {
+                    const uint64_t g = g0 + (uint64_t)u * nt;
+                    valid[u] = g < ngroups;
+                    if (valid[u]) acc[u] = p[gbase + g];
+                }
             }
-            for (int r = 1; r < P; r++) {
-                const Vt* p = (const Vt*)(uintptr_t)A.ins_w[(uint64_t)r * A.stride_words];
-                Vt x[U];
+            for (int rb = 1; rb < P; rb += B) {
+                const Vt* p[B];
 #pragma unroll
-                for (int u = 0; u < U; u++)
-                    if (valid[u]) x[u] = p[(lo / V) + g0 + (uint64_t)u * nt];
+                for (int k = 0; k < B; k++)
+                    p[k] = rb + k < P ? (const Vt*)(uintptr_t)A.ins_w[(uint64_t)(rb + k) * A.stride_words] : nullptr;
+                Vt x[B][U];
 #pragma unroll
-                for (int u = 0; u < U; u++)
-                    if (valid[u]) {
+                for (int k = 0; k < B; k++)
 #pragma unroll
-                        for (int k = 0; k < V; k++) acc[u].v[k] = f(x[u].v[k], acc[u].v[k]);
+                    for (int u = 0; u < U; u++)
+                        if (p[k] && valid[u]) x[k][u] = p[k][gbase + g0 + (uint64_t)u * nt];
+#pragma unroll
+                for (int k = 0; k < B; k++)
+                    if (p[k]) {
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                            if (valid[u]) {
+#pragma unroll
+                                for (int e = 0; e < V; e++) acc[u].v[e] = f(x[k][u].v[e], acc[u].v[e]);
+                            }
                     }
             }
-            const int r0 = A.root >= 0 ? A.root : 0;
-            const int r1 = A.root >= 0 ? A.root + 1 : P;
             for (int r = r0; r < r1; r++) {
                 Vt* o = (Vt*)(uintptr_t)A.outs_w[(uint64_t)r * A.stride_words];
 #pragma unroll
                 for (int u = 0; u < U; u++)
-                    if (valid[u]) o[(lo / V) + g0 + (uint64_t)u * nt] = acc[u];
+                    if (valid[u]) o[gbase + g0 + (uint64_t)u * nt] = acc[u];
             }
         }
         lo += ngroups * V;
     }
     for (uint64_t e = lo + (uint64_t)t; e < hi; e += (uint64_t)nt) {
         T acc = ((const T*)(uintptr_t)A.ins_w[0])[e];
-        for (int r = 1; r < P; r++) acc = f(((const T*)(uintptr_t)A.ins_w[(uint64_t)r * A.stride_words])[e], acc);
-        const int r0 = A.root >= 0 ? A.root : 0;
-        const int r1 = A.root >= 0 ? A.root + 1 : P;
+        for (int rb = 1; rb < P; rb += B) {
+            T x[B];
+#pragma unroll
+            for (int k = 0; k < B; k++)
+                if (rb + k < P) x[k] = ((const T*)(uintptr_t)A.ins_w[(uint64_t)(rb + k) * A.stride_words])[e];
+#pragma unroll
+            for (int k = 0; k < B; k++)
+                if (rb + k < P) acc = f(x[k], acc);
+        }
         for (int r = r0; r < r1; r++) ((T*)(uintptr_t)A.outs_w[(uint64_t)r * A.stride_words])[e] = acc;
     }
 }

@@ -401,11 +401,12 @@ __device__ inline void slot_take(int src, uint32_t seq, void* dstv, uint64_t cap
 
 // Copy a rendezvous message out of the sender's buffer (every wave of the
 // rank), then release the sender's completion word.
-__device__ inline __attribute__((noinline)) void rendezvous_copy(void* dst, const Taken& t) {
+__device__ inline __attribute__((noinline)) void rendezvous_copy(void* dst, uint64_t src, uint64_t n,
+                                                                 uint64_t done, uint32_t done_val) {
     fence_acquire_agent();  // pairs with the sender's release before it posted
-    rank_copy(dst, (const void*)(uintptr_t)t.rv_src, t.got);
+    rank_copy(dst, (const void*)(uintptr_t)src, n);
     fence_release_agent();
-    if (elected()) st_relaxed_agent((uint32_t*)(uintptr_t)t.rv_done, t.rv_val);
+    if (elected()) st_relaxed_agent((uint32_t*)(uintptr_t)done, done_val);
 }
 
 // Deliver message `seq` of channel src->me to receive request `idx`.  A
@@ -589,12 +590,7 @@ __device__ inline int earlier_taker(int pos, const Found& f) {
 __device__ inline __attribute__((noinline)) void finish_rendezvous(int idx) {
     RankLds* L = ctx();
     Req* q = &reqs()[idx];
-    Taken t;
-    t.got = q->got;
-    t.rv_src = q->rv_src;
-    t.rv_done = q->rv_done;
-    t.rv_val = q->rv_val;
-    rendezvous_copy((void*)(uintptr_t)q->buf, t);
+    rendezvous_copy((void*)(uintptr_t)q->buf, q->rv_src, q->got, q->rv_done, q->rv_val);
     q->state = RS_COMPLETE;
     L->pending_copies--;
 }
@@ -754,7 +750,7 @@ __device__ inline int recv_direct(void* buf, uint64_t cap, int src, int tag, int
         if (scan_channel(src, ctxid, tag, &f)) {
             Taken t;
             slot_take(f.src, f.seq, buf, cap, f.has_pre, f.pre, &t);
-            if (t.rndv) rendezvous_copy(buf, t);
+            if (t.rndv) rendezvous_copy(buf, t.rv_src, t.got, t.rv_done, t.rv_val);
             if (st) {
                 st->MPI_SOURCE = comm == MPI_COMM_WORLD ? src : comm_rank_of_world(comm_get(comm), src);
                 st->MPI_TAG = env_tag(f.env);

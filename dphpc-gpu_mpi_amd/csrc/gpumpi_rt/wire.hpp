@@ -92,7 +92,7 @@ __device__ inline void quantise_part(const QuantArgs& A, int t, int nt) {
 }
 
 struct WireFoldArgs {
-    const CollSlot* tab;  // SW_SBUF = wire image, SW_RBUF = FP32 output
+    const uint64_t* mem;  // members_lds(): [2r] wire image, [2r+1] FP32 output
     int nmembers;
     int fmt;
     uint64_t n;  // elements of the whole vector
@@ -103,12 +103,23 @@ __device__ __forceinline__ float wire_load(int fmt, const uint8_t* w, uint64_t n
     const float* scales = (const float*)(w + ((n + 15) & ~(uint64_t)15));
     return e4m3_to_f32(w[e]) * scales[e / kFp8Block];
 }
+// Rank-order FP32 fold of the dequantised members, kFoldBatch members'
+// loads in flight at a time.
 __device__ inline void wire_fold_part(const WireFoldArgs& A, int t, int nt) {
+    constexpr int B = kFoldBatch;
+    const int P = A.nmembers;
     for (uint64_t e = A.lo + (uint64_t)t; e < A.hi; e += (uint64_t)nt) {
-        float acc = wire_load(A.fmt, (const uint8_t*)(uintptr_t)A.tab[0].w[SW_SBUF], A.n, e);
-        for (int r = 1; r < A.nmembers; r++)
-            acc = wire_load(A.fmt, (const uint8_t*)(uintptr_t)A.tab[r].w[SW_SBUF], A.n, e) + acc;
-        for (int r = 0; r < A.nmembers; r++) ((float*)(uintptr_t)A.tab[r].w[SW_RBUF])[e] = acc;
+        float acc = wire_load(A.fmt, (const uint8_t*)(uintptr_t)A.mem[0], A.n, e);
+        for (int rb = 1; rb < P; rb += B) {
+            float x[B];
+#pragma unroll
+            for (int k = 0; k < B; k++)
+                if (rb + k < P) x[k] = wire_load(A.fmt, (const uint8_t*)(uintptr_t)A.mem[2 * (rb + k)], A.n, e);
+#pragma unroll
+            for (int k = 0; k < B; k++)
+                if (rb + k < P) acc = x[k] + acc;
+        }
+        for (int r = 0; r < P; r++) ((float*)(uintptr_t)A.mem[2 * r + 1])[e] = acc;
     }
 }
 
@@ -136,7 +147,8 @@ __device__ inline int wire_allreduce(const void* sendbuf, void* recvbuf, int cou
     comm_barrier(cc.c, cc.size);
     const uint64_t g0 = n * (uint64_t)cc.me / (uint64_t)cc.size, g1 = n * (uint64_t)(cc.me + 1) / (uint64_t)cc.size;
     if (g1 > g0) {
-        WireFoldArgs f{cc.tab, cc.size, fmt, n, g0, g1};
+        load_members(cc, SW_SBUF, SW_RBUF);
+        WireFoldArgs f{members_lds(), cc.size, fmt, n, g0, g1};
         par_run([=] __device__(int t, int nt) { wire_fold_part(f, t, nt); },
                            (g1 - g0) * (uint64_t)(cc.size) * 6);
     }

@@ -80,6 +80,9 @@ struct WorldAlloc {
         f(rank_stats); f(dworld); f(world_tables); f(saves);
         if (abort_word) (void)hipHostFree(abort_word);
         if (err) (void)hipHostFree(err);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
         *this = WorldAlloc();
     }
 };

@@ -55,6 +55,11 @@ inline T gpumpi_host_readfirstlane(T) { gpumpi_host_unsupported("readfirstlane")
 #define __builtin_amdgcn_readfirstlane(v) gpumpi_host_readfirstlane(v)
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 #define __builtin_amdgcn_s_sleep(n) ((void)0)
+// v_alignbyte_b32: bytes [r, r+4) of the 8-byte concatenation lo:hi (hi is the upper half).
+inline unsigned gpumpi_host_alignbyte(unsigned hi, unsigned lo, unsigned r) {
+    return (unsigned)(((((unsigned long long)hi) << 32) | lo) >> (8 * (r & 3)));
+}
+#define __builtin_amdgcn_alignbyte(hi, lo, r) gpumpi_host_alignbyte((hi), (lo), (r))
 #define __builtin_readsteadycounter() ((uint64_t)0)
 #define __builtin_amdgcn_is_private(p) false
 #define __builtin_amdgcn_is_shared(p) false

@@ -25,7 +25,15 @@ int main(int argc, char** argv) {
     // Per-member counts (elements): the block test's shapes.
     //   ALLTOALL / ALLGATHER: blk everywhere; ALLGATHERV: member r sends blk + r;
     //   ALLTOALLV: counts[s][d] = (s + 2 * d) % 5 + blk.
-    auto a2av = [&](int s, int d) { return (s + 2 * d) % 5 + blk; };
+    std::vector<int> cfile;
+    if (argc > 4) {  // ALLTOALLV counts[src * P + dst] from a file of P*P int32
+        FILE* f = std::fopen(argv[4], "rb");
+        if (!f) return 2;
+        cfile.resize((size_t)P * P);
+        if (std::fread(cfile.data(), 4, cfile.size(), f) != cfile.size()) return 2;
+        std::fclose(f);
+    }
+    auto a2av = [&](int s, int d) { return cfile.empty() ? (s + 2 * d) % 5 + blk : cfile[(size_t)s * P + d]; };
     std::vector<int*> sb(P), rb(P), scnt(P), sdsp(P), rcnt(P), rdsp(P);
     std::vector<uint64_t> sbytes(P), rbytes(P);
     std::vector<CollSlot> tab(P);
@@ -56,8 +64,10 @@ int main(int argc, char** argv) {
         }
         sbytes[r] = sn * es;
         rbytes[r] = rn * es;
-        sb[r] = (int*)std::malloc(sbytes[r] ? sbytes[r] : 1);
-        rb[r] = (int*)std::malloc(rbytes[r] ? rbytes[r] : 1);
+        // Whole 16-byte blocks: copies read aligned 16-byte source blocks
+        // (never past a block that holds a source byte).
+        sb[r] = (int*)aligned_alloc(16, (sbytes[r] + 16) / 16 * 16);
+        rb[r] = (int*)aligned_alloc(16, (rbytes[r] + 16) / 16 * 16);
         for (uint64_t i = 0; i < sn; i++) sb[r][i] = r * 100000 + (int)i;
         std::memset(rb[r], 0xff, rbytes[r]);
         // What coll_segments publishes for member r.

@@ -4,6 +4,7 @@ lane at a time (tests/hostbuild/hip/hip_runtime.h):
 * the all-to-all / all-gather data phase (coll.hpp seg_info, run_segments,
   copy_part) under AddressSanitizer, for every segment kind, small and large
   segments, so every address the device would touch is bounds-checked;
+* copy_part alone over random alignments (its realigning 16-byte path);
 * the reduction fold (ops.hpp fold_builtin) for all 216 legal
   (datatype, op) pairs, bit-exact with the oracle's rank-order fold, with
   and without the 16-byte vector path, for Allreduce and rooted Reduce."""
@@ -41,6 +42,24 @@ def segcheck():
         pytest.skip("clang++ not available")
     return _build("segments_check.cpp", os.path.join(OUT, "segments_check"),
                   ["-O1", "-fsanitize=address,undefined", "-fno-omit-frame-pointer"])
+
+
+@pytest.fixture(scope="module")
+def copycheck():
+    if not os.path.exists(CLANG):
+        pytest.skip("clang++ not available")
+    return _build("copy_check.cpp", os.path.join(OUT, "copy_check"),
+                  ["-O1", "-fsanitize=address,undefined", "-fno-omit-frame-pointer"])
+
+
+def test_copy_part_alignments_in_bounds(copycheck):
+    """copy_part over random source/destination alignments and lengths: exact
+    bytes, nothing written outside the destination, no source read outside
+    the aligned 16-byte blocks that hold source bytes (poisoned under ASan)."""
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
+    r = subprocess.run([copycheck, "3000"], capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+    assert "mismatches=0" in r.stdout
 
 
 @pytest.fixture(scope="module")
