@@ -73,6 +73,50 @@ def measured_traffic(P, S, threads, steps):
     return round(rec["hbm_bytes_per_step"] * steps)
 
 
+def copy_calibration(nbytes):
+    """Two launches of a plain dwordx4 copy of nbytes (read nbytes, write
+    nbytes): what FETCH_SIZE / WRITE_SIZE read for it scales the ring's."""
+    import gpumpi as gm
+    gm.load()
+    nbytes = nbytes // 16 * 16
+    a = gm.DeviceBuffer(nbytes)
+    b = gm.DeviceBuffer(nbytes)
+    a.zero()
+    for _ in range(2):
+        gm.copy_calibration(b, a, nbytes)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 2}), flush=True)
+    return 0
+
+
+def side_measurements(gm, np):
+    """Same-run figures for the other headline rows: exact FP32 Allreduce
+    (256 ranks x 16 MiB, algorithmic 2*P*N*4 bytes) and NPB IS class B on
+    128 ranks (keys/s, and its Alltoallv key exchange, 2 * 2^25 * 4 bytes
+    per iteration)."""
+    out = {}
+    P, n, iters = 256, 1 << 22, 10
+    ib = gm.DeviceBuffer(4 * n * P)
+    ob = gm.DeviceBuffer(4 * n * P)
+    ib.zero()
+    times = gm.DeviceBuffer(8 * P)
+    gm.run("coll_reduce", P, [0, gm.DT["MPI_FLOAT"], gm.OP["MPI_SUM"], n, ib, ob, 4 * n, -1, 0, 1, iters, times, 0, 0],
+           threads_per_rank=256, timeout_s=300.0)
+    t = float(times.download(np.float64).max()) / iters
+    out["allreduce_f32_256x16MiB_gbps"] = round(2.0 * P * n * 4 / t / 1e9, 1)
+    del ib, ob
+    res = gm.DeviceBuffer(8 * 8)
+    res.zero()
+    tout = gm.DeviceBuffer(8 * 4)
+    gm.run("npb_is", 128, [0, ord("B"), res, tout, 0, 10], threads_per_rank=256, user_lds_bytes=4096,
+           heap_bytes=128 * (32 << 20), max_requests=64, timeout_s=120.0)
+    r = res.download(np.int64)
+    tt = tout.download(np.float64)
+    out["npb_is_class_b_128_ranks"] = {
+        "verified": bool(r[0] == 50 + 128), "mkeys_per_s": round(10 * (1 << 25) / tt[0] / 1e6, 1),
+        "alltoallv_gbps": round(2 * 10 * (1 << 25) * 4 / tt[1] / 1e9, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,11 +126,16 @@ def main():
     ap.add_argument("--bytes", type=int, default=64 << 20)
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the Allreduce / NPB IS side measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (reference for HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the whole MPI world lives in one device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.bytes)
 
@@ -155,6 +204,8 @@ def main():
         "ring_gbps_mpi_wtime": round(wtime_gbps, 2),
         "payload_mismatches": bad,
     }
+    if not args.no_extras:
+        line.update(side_measurements(gm, np))
     print(json.dumps(line), flush=True)
     return 0 if bad == 0 else 1
 
