@@ -117,7 +117,7 @@ def side_measurements(gm, np):
     return out
 
 
-def main():
+def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the Allreduce / NPB IS side measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (reference for HBM counters) and exit")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the whole MPI world lives in one device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -207,6 +207,8 @@ def main():
     if not args.no_extras:
         line.update(side_measurements(gm, np))
     print(json.dumps(line), flush=True)
+    del sb, rb, ppb
+    gm.release_world()
     return 0 if bad == 0 else 1
 
 

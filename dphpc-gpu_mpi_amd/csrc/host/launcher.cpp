@@ -570,6 +570,14 @@ int gpumpi_reset_world(void) {
     return 0;
 }
 
+int gpumpi_release_world(void) {
+    if (g_world.valid) {
+        (void)hipDeviceSynchronize();
+        g_world.release();
+    }
+    return 0;
+}
+
 int gpumpi_device_alloc(void** ptr, size_t bytes) {
     HIPOK(hipMalloc(ptr, bytes ? bytes : 1));
     return 0;

@@ -36,7 +36,7 @@ OP = {n: i for i, n in enumerate(OPS)}
 CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpumpi_run_graph", "gpumpi_main",
                 "gpumpi_device_alloc", "gpumpi_device_free", "gpumpi_memcpy_htod",
                 "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error",
-                "gpumpi_reset_world"]
+                "gpumpi_reset_world", "gpumpi_release_world"]
 PROGS_SYMBOLS = ["gpumpi_prog_lookup", "gpumpi_prog_count", "gpumpi_prog_name",
                  "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal",
                  "gpumpi_copy_calibration"]
@@ -133,6 +133,11 @@ def load():
 
 def reset_world():
     load_core().gpumpi_reset_world()
+
+
+def release_world():
+    """Free the device world kept between launches."""
+    load_core().gpumpi_release_world()
 
 
 def last_error():

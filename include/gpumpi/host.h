@@ -108,6 +108,12 @@ int gpumpi_memset(void* dst, int value, size_t bytes);
  * cursors carry over between clean launches of the same world shape). */
 int gpumpi_reset_world(void);
 
+/* Free the device world kept between launches (channels, heaps, staging,
+ * stream); the next launch allocates it again.  For hosts that finish with
+ * the runtime before the process ends.  No counterpart in the reference,
+ * whose host main() exits after its one launch (main.cu:133-321). */
+int gpumpi_release_world(void);
+
 /* Largest rank count that can be co-resident for a configuration. */
 int gpumpi_max_ranks(const gpumpi_config* cfg, const void* kernel);
 
