@@ -22,11 +22,20 @@ __device__ __forceinline__ bool is_global_ptr(const void* p) {
 
 constexpr int kCopyUnroll = 4;
 
+// Global-memory (HBM) pointers: loads and stores through them are global_*
+// instructions, which count on vmcnt alone and cannot alias the stack.  (The
+// host build of this header for the sanitizer checks predefines it empty.)
+#ifndef GPUMPI_GLOBAL
+#define GPUMPI_GLOBAL __attribute__((address_space(1)))
+#endif
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16-byte vector, plain value type
+typedef GPUMPI_GLOBAL u32x4 gvec4;
+
 // Bytes [4*Q + r, 4*Q + r + 16) of the 32-byte concatenation lo:hi.
 template <int Q>
-__device__ __forceinline__ uint4 realign16(const uint4& lo, const uint4& hi, uint32_t r) {
+__device__ __forceinline__ u32x4 realign16(const u32x4 lo, const u32x4 hi, uint32_t r) {
     const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-    uint4 o;
+    u32x4 o;
     o.x = __builtin_amdgcn_alignbyte(w[Q + 1], w[Q], r);
     o.y = __builtin_amdgcn_alignbyte(w[Q + 2], w[Q + 1], r);
     o.z = __builtin_amdgcn_alignbyte(w[Q + 3], w[Q + 2], r);
@@ -34,22 +43,42 @@ __device__ __forceinline__ uint4 realign16(const uint4& lo, const uint4& hi, uin
     return o;
 }
 
+// d4[i] = s4[i] for i < nvec, worker t of nt taking i = t, t + nt, ...; four
+// loads in flight per lane before the four stores.  The four values are
+// named scalars, not an array: an array here was kept in scratch (three
+// dead 16-byte stack stores per iteration) and its stores forced a wait
+// after every load.
+template <typename DP, typename SP>
+__device__ __forceinline__ void copy_aligned(DP d4, SP s4, uint64_t nvec, int t, int nt) {
+    static_assert(kCopyUnroll == 4, "copy_aligned moves four vectors per step");
+    uint64_t i = (uint64_t)t;
+    const uint64_t st = (uint64_t)nt;
+    for (; i + 3 * st < nvec; i += 4 * st) {
+        const u32x4 v0 = s4[i], v1 = s4[i + st], v2 = s4[i + 2 * st], v3 = s4[i + 3 * st];
+        d4[i] = v0;
+        d4[i + st] = v1;
+        d4[i + 2 * st] = v2;
+        d4[i + 3 * st] = v3;
+    }
+    for (; i < nvec; i += st) d4[i] = s4[i];
+}
+
 // d4[i] = source bytes [16 i + 4 Q + r, +16) for i < nvec, reading only the
 // 16-byte aligned source blocks a4[i] and a4[i + 1]: every block read holds
 // at least one byte of the source range, so no read leaves its page.
 template <int Q>
-__device__ inline void copy_realigned(uint4* d4, const uint4* a4, uint64_t nvec, uint32_t r, int t, int nt) {
+__device__ inline void copy_realigned(gvec4* d4, const gvec4* a4, uint64_t nvec, uint32_t r, int t, int nt) {
     uint64_t i = (uint64_t)t;
     const uint64_t st = (uint64_t)nt;
-    for (; i + (kCopyUnroll - 1) * st < nvec; i += kCopyUnroll * st) {
-        uint4 lo[kCopyUnroll], hi[kCopyUnroll];
-#pragma unroll
-        for (int u = 0; u < kCopyUnroll; u++) {
-            lo[u] = a4[i + u * st];
-            hi[u] = a4[i + u * st + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < kCopyUnroll; u++) d4[i + u * st] = realign16<Q>(lo[u], hi[u], r);
+    for (; i + 3 * st < nvec; i += 4 * st) {
+        const u32x4 l0 = a4[i], h0 = a4[i + 1];
+        const u32x4 l1 = a4[i + st], h1 = a4[i + st + 1];
+        const u32x4 l2 = a4[i + 2 * st], h2 = a4[i + 2 * st + 1];
+        const u32x4 l3 = a4[i + 3 * st], h3 = a4[i + 3 * st + 1];
+        d4[i] = realign16<Q>(l0, h0, r);
+        d4[i + st] = realign16<Q>(l1, h1, r);
+        d4[i + 2 * st] = realign16<Q>(l2, h2, r);
+        d4[i + 3 * st] = realign16<Q>(l3, h3, r);
     }
     for (; i < nvec; i += st) d4[i] = realign16<Q>(a4[i], a4[i + 1], r);
 }
@@ -59,7 +88,8 @@ __device__ inline void copy_realigned(uint4* d4, const uint4* a4, uint64_t nvec,
 // dst == src.  The destination is aligned to 16 bytes by a short head, then
 // moved in 16-byte vectors (dwordx4 loads and stores, several in flight per
 // lane); a source at another alignment is read as aligned vectors and
-// shifted into place.
+// shifted into place.  Global (HBM) buffers take global_* instructions;
+// a private or LDS buffer takes the generic (flat) path.
 __device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t, int nt) {
     uint8_t* dst = (uint8_t*)dstv;
     const uint8_t* src = (const uint8_t*)srcv;
@@ -68,36 +98,29 @@ __device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t
     if (head > n) head = n;
     if ((uint64_t)t < head) dst[t] = src[t];
     const uint64_t nvec = (n - head) >> 4;
-    uint4* d4 = (uint4*)(dst + head);
+    uint8_t* d2 = dst + head;
     const uint8_t* s2 = src + head;
     const uint32_t m = (uint32_t)((uintptr_t)s2 & 15);
-    if (m != 0 && !is_global_ptr(src)) {
-        // Private or LDS source at another alignment: plain bytes (whole
-        // aligned blocks are read only from global memory, where a block
-        // never leaves its page).
-        for (uint64_t i = head + (uint64_t)t; i < n; i += (uint64_t)nt) dst[i] = src[i];
-        return;
-    }
-    if (m == 0) {
-        const uint4* s4 = (const uint4*)s2;
-        uint64_t i = (uint64_t)t;
-        const uint64_t st = (uint64_t)nt;
-        for (; i + (kCopyUnroll - 1) * st < nvec; i += kCopyUnroll * st) {
-            uint4 v[kCopyUnroll];
-#pragma unroll
-            for (int u = 0; u < kCopyUnroll; u++) v[u] = s4[i + u * st];
-#pragma unroll
-            for (int u = 0; u < kCopyUnroll; u++) d4[i + u * st] = v[u];
-        }
-        for (; i < nvec; i += st) d4[i] = s4[i];
-    } else {
-        const uint4* a4 = (const uint4*)(s2 - m);
+    const bool global = is_global_ptr(src) && is_global_ptr(dst);
+    if (m == 0 && global) {
+        copy_aligned((gvec4*)d2, (const gvec4*)s2, nvec, t, nt);
+    } else if (m == 0) {
+        copy_aligned((u32x4*)d2, (const u32x4*)s2, nvec, t, nt);
+    } else if (global) {
+        gvec4* d4 = (gvec4*)d2;
+        const gvec4* a4 = (const gvec4*)(s2 - m);
         switch (m >> 2) {
         case 0: copy_realigned<0>(d4, a4, nvec, m & 3, t, nt); break;
         case 1: copy_realigned<1>(d4, a4, nvec, m & 3, t, nt); break;
         case 2: copy_realigned<2>(d4, a4, nvec, m & 3, t, nt); break;
         default: copy_realigned<3>(d4, a4, nvec, m & 3, t, nt); break;
         }
+    } else {
+        // Private or LDS buffer at another alignment: plain bytes (whole
+        // aligned blocks are read only from global memory, where a block
+        // never leaves its page).
+        for (uint64_t i = head + (uint64_t)t; i < n; i += (uint64_t)nt) dst[i] = src[i];
+        return;
     }
     const uint64_t done = head + (nvec << 4);
     if ((uint64_t)t < n - done) dst[done + t] = src[done + t];

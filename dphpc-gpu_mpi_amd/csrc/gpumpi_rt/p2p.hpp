@@ -5,13 +5,16 @@
 // Posted/Completed Send/Recv, test, wait) with lock-free per-(src,dst)
 // channels in HBM:
 //
-//   * a channel is a ring of Q slots written only by the sender; a slot's
-//     envelope word (sequence, context, tag) is stored last with release
-//     semantics, so a receiver that sees the sequence sees the slot;
+//   * a channel is a ring of Q slots written only by the sender; every
+//     8-byte word of a slot carries the message's sequence number + 1 in
+//     its upper half, so the receiver validates each word by itself and
+//     an eager message needs no fence on either side;
 //   * messages up to the eager size travel inside the slot (the send
-//     completes at once); larger ones carry the sender's buffer address and
-//     the receiver copies straight from it, then releases the sender's
-//     completion word (rendezvous);
+//     completes at once); larger ones carry the sender's buffer address
+//     (released by the sender before the slot words, acquired by the
+//     receiver before its copy) and the receiver copies straight from it
+//     in MPI_Wait/MPI_Test, then releases the sender's completion word
+//     (rendezvous);
 //   * the receiver matches: it scans the slots of one channel (or of every
 //     member's channel for MPI_ANY_SOURCE) with one lane per slot and a
 //     wavefront ballot, taking the oldest matching message -- MPI's
