@@ -147,7 +147,7 @@ __device__ inline void run_cmd(const Cmd& c, int t, int nt) {
         copy_part((void*)(uintptr_t)c.a[0], (const void*)(uintptr_t)c.a[1], c.a[2], t, nt);
         break;
     case CMD_REDUCE: {
-        const uint64_t* mem = (const uint64_t*)(uintptr_t)c.a[0];  // members_lds()
+        const uint64_t* mem = (const uint64_t*)(uintptr_t)c.a[0];  // members_tab()
         FoldArgs A;
         A.ins_w = &mem[0];
         A.outs_w = &mem[1];
@@ -175,8 +175,12 @@ __device__ inline void run_cmd(const Cmd& c, int t, int nt) {
 
 // Run a command on wave 0 alone when it is small, on every wave otherwise.
 __device__ inline void run_sized(const Cmd& c, uint64_t bytes_moved) {
-    if (bytes_moved >= (uint64_t)ctx()->helper_min_bytes && ctx()->waves > 1) wg_run(c);
-    else run_cmd(c, lane_id(), 64);
+    if (bytes_moved >= (uint64_t)ctx()->helper_min_bytes && ctx()->waves > 1) {
+        wg_run(c);
+    } else {
+        run_cmd(c, lane_id(), 64);
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -205,13 +209,14 @@ __device__ inline void publish(CollCtx& cc, int word, uint64_t v) {
 __device__ inline uint64_t peer_word(CollCtx& cc, int r, int word) { return uniform_u64(cc.tab[r].w[word]); }
 
 // Copy every member's `word_in` and `word_out` table entries into
-// members_lds() (wave 0; the helpers see it through the command handoff).
+// members_tab() (wave 0; the helpers see it through the command handoff).
 __device__ inline void load_members(CollCtx& cc, int word_in, int word_out) {
-    uint64_t* m = members_lds();
+    uint64_t* m = members_tab();
     for (int r = lane_id(); r < cc.size; r += 64) {
         m[2 * r] = cc.tab[r].w[word_in];
         m[2 * r + 1] = cc.tab[r].w[word_out];
     }
+    wave_sync();
 }
 
 // Are the given words of every member 16-byte aligned?
@@ -252,7 +257,7 @@ __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int coun
     if (i_out && r_eff != recvbuf && src == recvbuf) r_eff = (void*)s_eff;  // in place, staged
     publish(cc, SW_SBUF, (uint64_t)(uintptr_t)s_eff);
     publish(cc, SW_RBUF, (uint64_t)(uintptr_t)r_eff);
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     // My slice: groups of 16 bytes, split evenly over the members.
     const int V = es >= 16 ? 1 : 16 / es;
     const uint64_t groups = ((uint64_t)count + V - 1) / V;
@@ -269,13 +274,13 @@ __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int coun
         c.i3 = all ? -1 : root;
         c.i4 = all_aligned16(cc, SW_SBUF, SW_RBUF, all ? -1 : root) ? 1 : 0;
         load_members(cc, SW_SBUF, SW_RBUF);
-        c.a[0] = (uint64_t)(uintptr_t)members_lds();
+        c.a[0] = (uint64_t)(uintptr_t)members_tab();
         c.a[1] = lo;
         c.a[2] = hi;
         const uint64_t moved = (hi - lo) * (uint64_t)es * (uint64_t)(all ? 2 * cc.size : cc.size + 1);
         run_sized(c, moved);
     }
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     if (i_out) stage_back(recvbuf, r_eff, bytes);
     return MPI_SUCCESS;
 }
@@ -291,9 +296,9 @@ __device__ inline int coll_bcast(void* buf, int count, MPI_Datatype dt, int root
     if (cc.size == 1) return MPI_SUCCESS;
     CollStage stg;
     if (cc.me == root) publish(cc, SW_SBUF, (uint64_t)(uintptr_t)stg.in(buf, bytes));
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     if (cc.me != root) rank_copy(buf, (const void*)(uintptr_t)peer_word(cc, root, SW_SBUF), bytes);
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     return MPI_SUCCESS;
 }
 
@@ -329,7 +334,7 @@ __device__ inline int coll_gather(bool v, const void* sendbuf, int sendcount, MP
         publish(cc, SW_RBUF, (uint64_t)(uintptr_t)r_eff);
         publish(cc, SW_AUX, (uint64_t)res);
     }
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     const bool in_place = at_root && sendbuf == MPI_IN_PLACE;
     if (!in_place) {
         uint8_t* rb = (uint8_t*)(uintptr_t)peer_word(cc, root, SW_RBUF);
@@ -343,7 +348,7 @@ __device__ inline int coll_gather(bool v, const void* sendbuf, int sendcount, MP
         }
         rank_copy(rb + off, sendbuf, (uint64_t)sendcount * (uint64_t)ses);
     }
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     if (at_root) stage_back(recvbuf, r_eff, rtotal);
     return MPI_SUCCESS;
 }
@@ -378,7 +383,7 @@ __device__ inline int coll_scatter(bool v, const void* sendbuf, int sendcount, c
         publish(cc, SW_SBUF, (uint64_t)(uintptr_t)stg.in(sendbuf, stotal));
         publish(cc, SW_AUX, (uint64_t)ses);
     }
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     if (!(at_root && recvbuf == MPI_IN_PLACE)) {
         const uint8_t* sb = (const uint8_t*)(uintptr_t)peer_word(cc, root, SW_SBUF);
         const uint64_t ses = peer_word(cc, root, SW_AUX);
@@ -396,7 +401,7 @@ __device__ inline int coll_scatter(bool v, const void* sendbuf, int sendcount, c
         if (n > cap) return GPUMPI_FAIL(MPI_ERR_TRUNCATE, "scatter block larger than the receive buffer");
         rank_copy(recvbuf, sb + off, n);
     }
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     return MPI_SUCCESS;
 }
 
@@ -468,7 +473,7 @@ __device__ inline int coll_segments(int kind, const void* sendbuf, int sendcount
     }
     publish(cc, SW_SBUF, (uint64_t)(uintptr_t)s_eff);
     publish(cc, SW_RBUF, (uint64_t)(uintptr_t)r_eff);
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     Cmd c{};
     c.op = CMD_ALLTOALLV;
     c.i0 = P;
@@ -480,7 +485,7 @@ __device__ inline int coll_segments(int kind, const void* sendbuf, int sendcount
     c.a[1] = (uint64_t)ses;
     c.a[2] = (uint64_t)res;
     run_sized(c, 2 * rtotal);
-    comm_barrier(cc.c, cc.size);
+    if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
     stage_back(recvbuf, r_eff, rtotal);
     return MPI_SUCCESS;
 }

@@ -106,7 +106,7 @@ __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int co
     tab[prank].w[0] = (uint64_t)(uint32_t)color;
     tab[prank].w[1] = (uint64_t)(uint32_t)key;
     tab[prank].w[2] = 0;
-    comm_barrier(parent, psize);
+    if (!comm_barrier(parent, psize)) return MPI_COMM_NULL;
     // New rank = members of my colour ordered before me by (key, parent rank);
     // leader = the member of my colour with the smallest (key, parent rank).
     int cnt_less = 0, cnt_size = 0;
@@ -160,7 +160,7 @@ __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int co
             tab[prank].w[2] = (uint64_t)(uintptr_t)nc;
         }
     }
-    comm_barrier(parent, psize);
+    if (!comm_barrier(parent, psize)) return MPI_COMM_NULL;
     MPI_Comm out = MPI_COMM_NULL;
     if (color != MPI_UNDEFINED) {
         CommShared* nc = (CommShared*)(uintptr_t)uniform_u64(tab[leader].w[2]);
@@ -170,7 +170,7 @@ __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int co
             out = (MPI_Comm)nc;
         }
     }
-    comm_barrier(parent, psize);
+    if (!comm_barrier(parent, psize)) return MPI_COMM_NULL;
     return out;
 }
 

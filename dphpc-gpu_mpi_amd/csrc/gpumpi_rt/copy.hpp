@@ -90,7 +90,7 @@ __device__ inline void copy_realigned(gvec4* d4, const gvec4* a4, uint64_t nvec,
 // lane); a source at another alignment is read as aligned vectors and
 // shifted into place.  Global (HBM) buffers take global_* instructions;
 // a private or LDS buffer takes the generic (flat) path.
-__device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t, int nt) {
+__device__ inline void copy_part_lanes(void* dstv, const void* srcv, uint64_t n, int t, int nt) {
     uint8_t* dst = (uint8_t*)dstv;
     const uint8_t* src = (const uint8_t*)srcv;
     if (n == 0 || dst == src) return;
@@ -124,6 +124,11 @@ __device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t
     }
     const uint64_t done = head + (nvec << 4);
     if ((uint64_t)t < n - done) dst[done + t] = src[done + t];
+}
+// The wave's lanes may read what the others copied as soon as this returns.
+__device__ inline void copy_part(void* dstv, const void* srcv, uint64_t n, int t, int nt) {
+    copy_part_lanes(dstv, srcv, n, t, nt);
+    wave_sync();
 }
 
 // Every lane copies all n bytes: used when dst is the caller's private

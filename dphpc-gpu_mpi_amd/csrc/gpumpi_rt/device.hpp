@@ -28,6 +28,12 @@ __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// Every lane of the wave reaches this point before any goes on.  Free on
+// the GPU, whose lanes run in lockstep (it only keeps the compiler from
+// moving memory accesses across it); it marks where lanes hand data to each
+// other through memory: lane-parallel stores that other lanes of the same
+// wave read next.
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 __device__ __forceinline__ uint64_t active_mask() { return __ballot(1); }
 // True on exactly one active lane (the lowest).
 __device__ __forceinline__ bool elected() {
@@ -143,6 +149,7 @@ struct RankLds {
     uint32_t* chan_head;
     uint32_t* done_words;
     CommShared* comm_world;
+    uint64_t* members;  // collective member table: LDS, or this rank's row of World::members_hbm
     int rank;       // world rank
     int nranks;
     int waves;
@@ -170,7 +177,7 @@ struct RankLds {
     uint64_t bounce_top; // staging bump offset for private rendezvous sends
     int bounce_users;
     // offsets of the variable arrays from the LDS base
-    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_members, off_user;
+    uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user;
     uint32_t lds_bytes;
 };
 
@@ -189,17 +196,16 @@ __device__ __forceinline__ uint64_t* free_bits() {
 }
 // Per-collective copy of the members' two buffer addresses (input, output),
 // [2 * member + k]: read once from the communicator's table, then by every
-// wave of the rank from LDS.
-__device__ __forceinline__ uint64_t* members_lds() {
-    return (uint64_t*)(gpumpi_lds_base + ctx()->off_members);
-}
+// wave of the rank -- from LDS, or, for worlds too large to keep it there
+// beside the per-peer channel arrays, from a rank-private HBM row.
+__device__ __forceinline__ uint64_t* members_tab() { return ctx()->members; }
 
 // Byte size of the LDS image for a configuration (host and device agree).
 __host__ __device__ inline uint32_t align_up(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
 struct LdsLayout {
     uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_members, off_user, total;
 };
-__host__ __device__ inline LdsLayout lds_layout(int nranks, int R, int user_bytes) {
+__host__ __device__ inline LdsLayout lds_layout(int nranks, int R, int user_bytes, bool members_in_lds) {
     LdsLayout L;
     uint32_t o = align_up((uint32_t)sizeof(RankLds), 16);
     L.off_reqs = o; o = align_up(o + (uint32_t)(sizeof(Req) * R), 16);
@@ -208,7 +214,7 @@ __host__ __device__ inline LdsLayout lds_layout(int nranks, int R, int user_byte
     L.off_tx = o; o = align_up(o + (uint32_t)(sizeof(TxChan) * nranks), 16);
     L.off_unmatched = o; o = align_up(o + (uint32_t)(2 * nranks), 16);
     L.off_freebits = o; o = align_up(o + (uint32_t)(8 * ((R + 63) / 64)), 16);
-    L.off_members = o; o = align_up(o + (uint32_t)(16 * nranks), 16);
+    L.off_members = o; o = align_up(o + (members_in_lds ? (uint32_t)(16 * nranks) : 0u), 16);
     L.off_user = o; o = align_up(o + (uint32_t)user_bytes, 16);
     L.total = o;
     return L;

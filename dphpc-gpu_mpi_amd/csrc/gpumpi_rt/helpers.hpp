@@ -41,11 +41,13 @@ __device__ inline void wg_run(const Cmd& c) {
     const int waves = L->waves;
     if (waves == 1) {
         run_cmd(c, lane_id(), 64);
+        wave_sync();
         return;
     }
     const uint32_t seq = wg_post(c);
     run_cmd(c, (int)threadIdx.x, waves * 64);
     wg_wait(seq);
+    wave_sync();
 }
 
 // A device functor run by every thread of the rank (CMD_USER): the functor
@@ -67,8 +69,12 @@ __device__ inline void par_all_waves(const F& f) {
 // helper_min_bytes of memory.
 template <typename F>
 __device__ inline void par_run(const F& f, uint64_t bytes) {
-    if (bytes < (uint64_t)ctx()->helper_min_bytes || ctx()->waves == 1) f(lane_id(), 64);
-    else par_all_waves(f);
+    if (bytes < (uint64_t)ctx()->helper_min_bytes || ctx()->waves == 1) {
+        f(lane_id(), 64);
+        wave_sync();
+    } else {
+        par_all_waves(f);
+    }
 }
 
 // Copy by this rank: wave 0 alone for small or private-destination copies,
@@ -137,6 +143,9 @@ __device__ inline void helpers_exit() {
 // gpu_libs/gpu_libc/cuda_mpi.cuh:519-521 and works on every communicator
 // (the reference's MPI_Barrier trapped on non-WORLD, mpi.cu:324-331).
 // ---------------------------------------------------------------------------
+// Returns false when the wait ended because the launch was aborted (an MPI
+// error on some rank, or the host watchdog): the members have then not all
+// arrived, so a collective must not read their table entries and returns.
 __device__ inline bool comm_barrier(CommShared* c, int size) {
     if (size <= 1) return true;
     uint64_t old = 0;

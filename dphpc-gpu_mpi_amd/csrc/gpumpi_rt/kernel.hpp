@@ -18,7 +18,7 @@ __device__ __forceinline__ int rank_of_block(const Config& cfg, int b) {
 
 __device__ inline void rank_init(World* w) {
     const Config& cfg = w->cfg;
-    const LdsLayout lay = lds_layout(cfg.nranks, cfg.max_requests, cfg.user_lds_bytes);
+    const LdsLayout lay = lds_layout(cfg.nranks, cfg.max_requests, cfg.user_lds_bytes, cfg.members_in_lds != 0);
     // Zero the whole LDS image with every thread.
     uint32_t* z = (uint32_t*)gpumpi_lds_base;
     for (uint32_t i = threadIdx.x; i < lay.total / 4; i += blockDim.x) z[i] = 0;
@@ -31,6 +31,8 @@ __device__ inline void rank_init(World* w) {
         L->done_words = w->done_words;
         L->comm_world = w->comm_world;
         L->rank = rank_of_block(cfg, (int)blockIdx.x);
+        L->members = cfg.members_in_lds ? (uint64_t*)(gpumpi_lds_base + lay.off_members)
+                                        : w->members_hbm + 2 * (uint64_t)cfg.nranks * (uint64_t)L->rank;
         L->nranks = cfg.nranks;
         L->waves = cfg.waves;
         L->threads = cfg.threads;
@@ -46,7 +48,6 @@ __device__ inline void rank_init(World* w) {
         L->off_tx = lay.off_tx;
         L->off_unmatched = lay.off_unmatched;
         L->off_freebits = lay.off_freebits;
-        L->off_members = lay.off_members;
         L->off_user = lay.off_user;
         L->lds_bytes = lay.total;
     }

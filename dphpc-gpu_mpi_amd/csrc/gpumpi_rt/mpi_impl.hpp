@@ -82,8 +82,7 @@ __device__ inline int irecv_common(void* buf, int count, MPI_Datatype dt, int so
 __device__ inline int MPI_Init(int*, char***) {
     using namespace gpumpi::rt;
     ctx()->initialized = 1;
-    comm_barrier(world()->comm_world, ctx()->nranks);
-    return MPI_SUCCESS;
+    return comm_barrier(world()->comm_world, ctx()->nranks) ? MPI_SUCCESS : MPI_ERR_OTHER;
 }
 __device__ inline int MPI_Init_thread(int* argc, char*** argv, int, int* provided) {
     if (provided) *provided = MPI_THREAD_FUNNELED;
@@ -95,9 +94,9 @@ __device__ inline int MPI_Initialized(int* flag) {
 }
 __device__ inline int MPI_Finalize(void) {
     using namespace gpumpi::rt;
-    comm_barrier(world()->comm_world, ctx()->nranks);
+    const bool ok = comm_barrier(world()->comm_world, ctx()->nranks);
     ctx()->finalized = 1;
-    return MPI_SUCCESS;
+    return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
 }
 __device__ inline int MPI_Abort(MPI_Comm, int errorcode) {
     gpumpi::rt::raise_error(errorcode ? errorcode : MPI_ERR_OTHER, __LINE__, "MPI_Abort called");

@@ -380,6 +380,7 @@ __device__ inline void slot_take(int src, uint32_t seq, void* dstv, uint64_t cap
                 if (w >= SF_PAYLOAD && w < need) slot_payload_put(dst, n, (uint64_t)(w - SF_PAYLOAD), val);
             }
         }
+        wave_sync();
     } else {
         const uint32_t val = slot_word_value(v);
         t->rv_src = ((uint64_t)lane_u32(val, SF_SRC_HI) << 32) | lane_u32(val, SF_SRC_LO);

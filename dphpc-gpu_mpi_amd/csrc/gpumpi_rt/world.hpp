@@ -40,6 +40,7 @@ struct Config {
     int lds_bytes;        // dynamic LDS per workgroup
     int spin_abort_period; // polls between abort-word checks
     int user_lds_bytes;    // LDS reserved for the rank program (gpumpi_user_lds)
+    int members_in_lds;    // 1: collective member tables in LDS; 0: in World::members_hbm
 };
 
 // Envelope word of a slot: [63:32] sequence+1, [31:16] context, [15:0] tag.
@@ -131,6 +132,9 @@ struct World {
     // restores them at start and saves them at exit.
     uint8_t* saves;        // [P * save_bytes]
     uint64_t save_bytes;
+    // Collective member tables (two words per member) when they do not fit
+    // in LDS beside the other per-peer arrays: [P * 2P], rank-private rows.
+    uint64_t* members_hbm;
 };
 
 // Per-rank save block: stamp, validity magic, then RxChan[P] and TxChan[P]

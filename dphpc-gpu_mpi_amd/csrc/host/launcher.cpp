@@ -68,6 +68,7 @@ struct WorldAlloc {
     World* dworld = nullptr;
     uint8_t* world_tables = nullptr;  // world_of, rank_of_world, slots of WORLD; self tables
     uint8_t* saves = nullptr;         // per-rank channel cursors between launches
+    uint64_t* members_hbm = nullptr;  // [P * 2P] collective member tables (large worlds)
     uint64_t save_bytes = 0;
     bool need_reset = true;           // next launch starts the channels from zero
     hipStream_t stream = nullptr;
@@ -77,7 +78,7 @@ struct WorldAlloc {
         auto f = [](void* p) { if (p) (void)hipFree(p); };
         f(slots); f(chan_head); f(done_words); f(staging); f(arena); f(arena_top);
         f(comm_world); f(comm_self); f(comm_pool); f(comm_pool_top); f(next_ctx); f(rank_ret);
-        f(rank_stats); f(dworld); f(world_tables); f(saves);
+        f(rank_stats); f(dworld); f(world_tables); f(saves); f(members_hbm);
         if (abort_word) (void)hipHostFree(abort_word);
         if (err) (void)hipHostFree(err);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -124,6 +125,7 @@ int alloc_world(const Config& cfg) {
     HIPOK(hipMalloc(&W.world_tables, tables));
     W.save_bytes = save_block_bytes(cfg.nranks);
     HIPOK(hipMalloc(&W.saves, P * W.save_bytes));
+    HIPOK(hipMalloc(&W.members_hbm, P * 2 * P * sizeof(uint64_t)));
     W.need_reset = true;
     HIPOK(hipHostMalloc((void**)&W.abort_word, 64, hipHostMallocCoherent));
     HIPOK(hipHostMalloc((void**)&W.err, sizeof(ErrorRecord), hipHostMallocCoherent));
@@ -206,6 +208,7 @@ int init_world(const Config& cfg, void* user_args) {
     hw.err = W.err;
     hw.user_args = user_args;
     hw.saves = W.saves;
+    hw.members_hbm = W.members_hbm;
     hw.save_bytes = W.save_bytes;
     HIPOK(hipMemcpyAsync(W.dworld, &hw, sizeof hw, hipMemcpyHostToDevice, s));
     HIPOK(hipStreamSynchronize(s));
@@ -256,7 +259,14 @@ int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
     if (g.wall_hz == 0) g.wall_hz = 100000000ull;
     g.num_xcd = (cus % 8 == 0 && cus >= 64) ? 8 : 1;
     g.user_lds_bytes = c.user_lds_bytes;
-    const LdsLayout lay = lds_layout(g.nranks, g.max_requests, g.user_lds_bytes);
+    // Collective member tables stay in LDS unless they would keep the ranks
+    // from being co-resident: ceil(P / CUs) rank workgroups share a CU's LDS.
+    int cu_lds = 0;
+    HIPOK(hipDeviceGetAttribute(&cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    const int per_cu = (g.nranks + cus - 1) / (cus > 0 ? cus : 1);
+    const LdsLayout with_members = lds_layout(g.nranks, g.max_requests, g.user_lds_bytes, true);
+    g.members_in_lds = (uint64_t)align_up(with_members.total, 1024) * (uint64_t)per_cu <= (uint64_t)cu_lds ? 1 : 0;
+    const LdsLayout lay = lds_layout(g.nranks, g.max_requests, g.user_lds_bytes, g.members_in_lds != 0);
     g.lds_bytes = (int)lay.total;
     *lds_bytes = (int)lay.total;
     int max_lds = 0;

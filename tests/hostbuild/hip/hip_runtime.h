@@ -40,6 +40,10 @@ extern thread_local gpumpi_host_idx threadIdx, blockIdx, blockDim, gridDim;
 struct alignas(16) uint4 {
     unsigned x, y, z, w;
 };
+struct alignas(16) float4 {
+    float x, y, z, w;
+};
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 
 inline unsigned __lane_id() { return (unsigned)gpumpi_host_lane; }
 inline unsigned long long __ballot(int) { gpumpi_host_unsupported("ballot"); }
@@ -56,6 +60,7 @@ inline T gpumpi_host_readfirstlane(T) { gpumpi_host_unsupported("readfirstlane")
 #define __builtin_amdgcn_readfirstlane(v) gpumpi_host_readfirstlane(v)
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 #define __builtin_amdgcn_s_sleep(n) ((void)0)
+#define __builtin_amdgcn_wave_barrier() ((void)0)
 // v_alignbyte_b32: bytes [r, r+4) of the 8-byte concatenation lo:hi (hi is the upper half).
 inline unsigned gpumpi_host_alignbyte(unsigned hi, unsigned lo, unsigned r) {
     return (unsigned)(((((unsigned long long)hi) << 32) | lo) >> (8 * (r & 3)));

@@ -1,0 +1,124 @@
+// CPU-simulated launches of the collective test programs (progs/coll_bench.hip)
+// with the shapes of the GPU tests, checked against the expected layouts.
+//   run_coll alltoallv COUNTS_FILE THREADS     (COUNTS_FILE: P, then P*P counts)
+//   run_coll blocks KIND THREADS               (P = 6, block 5, root 2)
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gpumpi/host.h"
+
+extern "C" const void* gpumpi_prog_lookup(const char* name);
+
+static int launch(const char* prog, int P, int threads, std::vector<uint64_t>& words) {
+    gpumpi_config cfg;
+    gpumpi_default_config(&cfg, P);
+    cfg.threads_per_rank = threads;
+    cfg.timeout_s = 0;
+    gpumpi_result res;
+    char name[64];
+    std::snprintf(name, sizeof name, "%s", prog);
+    char* av[] = {name, nullptr};
+    const int rc = gpumpi_run(&cfg, gpumpi_prog_lookup(prog), 1, av, words.data(), &res);
+    std::printf("%s P=%d rc=%d status=%d mpi_error=%d rank=%d line=%d msg='%s'\n", prog, P, rc, res.status,
+                res.mpi_error, res.mpi_error_rank, res.mpi_error_line, res.mpi_error_msg);
+    if (rc) std::printf("error: %s\n", gpumpi_last_error());
+    std::fflush(stdout);
+    return rc;
+}
+
+static int alltoallv(const char* file, int threads) {
+    FILE* f = std::fopen(file, "r");
+    if (!f) return 2;
+    int P = 0;
+    if (std::fscanf(f, "%d", &P) != 1) return 2;
+    std::vector<int> counts((size_t)P * P);
+    for (int& c : counts) if (std::fscanf(f, "%d", &c) != 1) return 2;
+    std::fclose(f);
+    std::vector<long> srow(P, 0), rcol(P, 0);
+    for (int s = 0; s < P; s++)
+        for (int d = 0; d < P; d++) {
+            srow[s] += counts[(size_t)s * P + d];
+            rcol[d] += counts[(size_t)s * P + d];
+        }
+    long smax = 0, rmax = 0;
+    for (int r = 0; r < P; r++) {
+        smax = std::max(smax, srow[r]);
+        rmax = std::max(rmax, rcol[r]);
+    }
+    const uint64_t sstride = ((uint64_t)smax * 4 + 255) / 256 * 256 + 256;
+    const uint64_t rstride = ((uint64_t)rmax * 4 + 255) / 256 * 256 + 256;
+    uint8_t *sbp = nullptr, *rbp = nullptr;
+    if (gpumpi_device_alloc((void**)&sbp, sstride * P) || gpumpi_device_alloc((void**)&rbp, rstride * P)) return 4;
+    std::memset(rbp, 0, rstride * P);
+    struct View { uint8_t* p; uint8_t* data() { return p; } } sb{sbp}, rb{rbp};
+    for (int r = 0; r < P; r++)
+        for (long i = 0; i < srow[r]; i++) ((int32_t*)(sb.data() + r * sstride))[i] = (int32_t)(r * 1000003 + i * 7919);
+    std::vector<uint64_t> words(64, 0);
+    words[1] = 2;  // MPI_INT (datatype enum order)
+    words[2] = (uint64_t)(uintptr_t)counts.data();
+    words[4] = (uint64_t)(uintptr_t)sb.data();
+    words[5] = (uint64_t)(uintptr_t)rb.data();
+    words[6] = sstride;
+    words[7] = rstride;
+    words[10] = 1;
+    if (launch("coll_alltoallv", P, threads, words)) return 1;
+    long bad = 0;
+    for (int r = 0; r < P; r++) {
+        long o = 0;
+        for (int s = 0; s < P; s++) {
+            long sd = 0;
+            for (int k = 0; k < r; k++) sd += counts[(size_t)s * P + k];
+            for (int i = 0; i < counts[(size_t)s * P + r]; i++, o++)
+                bad += ((int32_t*)(rb.data() + r * rstride))[o] != (int32_t)(s * 1000003 + (sd + i) * 7919);
+        }
+    }
+    std::printf("alltoallv mismatches: %ld\n", bad);
+    return bad ? 1 : 0;
+}
+
+static int blocks(int kind, int threads) {
+    const int P = 6, blk = 5, root = 2;
+    const uint64_t stride = 4 * 1024;
+    std::vector<uint8_t> sb(stride * P), rb(stride * P, 0);
+    for (int r = 0; r < P; r++)
+        for (int i = 0; i < 1024; i++) ((int32_t*)(sb.data() + r * stride))[i] = i + 1000 * r;
+    std::vector<uint64_t> words(64, 0);
+    words[1] = (uint64_t)kind;
+    words[2] = blk;
+    words[3] = root;
+    words[4] = (uint64_t)(uintptr_t)sb.data();
+    words[5] = (uint64_t)(uintptr_t)rb.data();
+    words[6] = stride;
+    words[7] = stride;
+    if (launch("coll_blocks", P, threads, words)) return 1;
+    auto got = [&](int r, int i) { return ((int32_t*)(rb.data() + r * stride))[i]; };
+    auto val = [&](int r, int i) { return i + 1000 * r; };
+    long bad = 0;
+    std::vector<int> cnt(P), dsp(P);
+    for (int r = 0; r < P; r++) {
+        cnt[r] = blk + r;
+        dsp[r] = r ? dsp[r - 1] + cnt[r - 1] : 0;
+    }
+    switch (kind) {
+    case 0: for (int r = 0; r < P; r++) for (int i = 0; i < blk; i++) bad += got(root, r * blk + i) != val(r, i); break;
+    case 1: for (int r = 0; r < P; r++) for (int i = 0; i < cnt[r]; i++) bad += got(root, dsp[r] + i) != val(r, i); break;
+    case 2: for (int r = 0; r < P; r++) for (int i = 0; i < blk; i++) bad += got(r, i) != val(root, r * blk + i); break;
+    case 3: for (int r = 0; r < P; r++) for (int i = 0; i < cnt[r]; i++) bad += got(r, i) != val(root, dsp[r] + i); break;
+    case 4: for (int d = 0; d < P; d++) for (int r = 0; r < P; r++) for (int i = 0; i < blk; i++) bad += got(d, r * blk + i) != val(r, i); break;
+    case 5: for (int d = 0; d < P; d++) for (int r = 0; r < P; r++) for (int i = 0; i < cnt[r]; i++) bad += got(d, dsp[r] + i) != val(r, i); break;
+    default: for (int d = 0; d < P; d++) for (int s = 0; s < P; s++) for (int i = 0; i < blk; i++) bad += got(d, s * blk + i) != val(s, d * blk + i); break;
+    }
+    std::printf("blocks kind %d mismatches: %ld\n", kind, bad);
+    return bad ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 4 && !std::strcmp(argv[1], "alltoallv")) return alltoallv(argv[2], std::atoi(argv[3]));
+    if (argc >= 4 && !std::strcmp(argv[1], "blocks")) return blocks(std::atoi(argv[2]), std::atoi(argv[3]));
+    std::fprintf(stderr, "usage: run_coll alltoallv COUNTS THREADS | blocks KIND THREADS\n");
+    return 2;
+}
