@@ -112,31 +112,50 @@ __device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_
     }
 }
 
+// Segment k's descriptor from lane `from` of the wave (wave-uniform result).
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int from) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, from);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), from);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The segments of a multi-segment copy are visited in the order
+// j = 1..P (peer k = (me + j) mod P, spreading the sources).  A wave first
+// loads the descriptors of up to 64 of its segments at once, one lane per
+// segment (one round trip of table, count and displacement loads instead
+// of one per segment), then copies them one after another.
+//   large segments: every worker of the rank on each segment in turn;
+//   small segments: wave w takes segments j = 1 + w, 1 + w + nw, ...
 __device__ inline void run_segments(const Cmd& c, int t, int nt) {
     const int P = c.i0, me = c.i1;
-    const int skip_self = c.i3;
-    if (c.i4) {
-        // Large segments: all workers on one segment at a time.
-        for (int j = 1; j <= P; j++) {
-            const int k = (me + j) % P;
-            if (k == me && skip_self) continue;
-            uint8_t* d;
-            const uint8_t* s;
-            uint64_t n;
-            seg_info(c, k, &d, &s, &n);
-            copy_part(d, s, n, t, nt);
+    const bool skip_self = c.i3 != 0;
+    const bool large = c.i4 != 0;
+    const int lane = t & 63;
+    const int first = large ? 1 : 1 + (t >> 6);
+    const int step = large ? 1 : (nt + 63) >> 6;
+    const int count = first <= P ? (P - first) / step + 1 : 0;
+    for (int b = 0; b < count; b += 64) {
+        // Lane l: descriptor of this wave's segment b + l.
+        uint64_t d = 0, sp = 0, n = 0;
+        const int i = b + lane;
+        if (i < count) {
+            const int k = (me + first + i * step) % P;
+            if (!(k == me && skip_self)) {
+                uint8_t* dd;
+                const uint8_t* ss;
+                seg_info(c, k, &dd, &ss, &n);
+                d = (uint64_t)(uintptr_t)dd;
+                sp = (uint64_t)(uintptr_t)ss;
+            }
         }
-    } else {
-        // Small segments: one wave per segment.
-        const int w = t >> 6, nw = (nt + 63) >> 6, l = t & 63;
-        for (int j = 1 + w; j <= P; j += nw) {
-            const int k = (me + j) % P;
-            if (k == me && skip_self) continue;
-            uint8_t* d;
-            const uint8_t* s;
-            uint64_t n;
-            seg_info(c, k, &d, &s, &n);
-            copy_part(d, s, n, l, 64);
+        const int nb = count - b < 64 ? count - b : 64;
+        for (int s2 = 0; s2 < nb; s2++) {
+            const uint64_t bytes = lane_u64(n, s2);
+            if (bytes == 0) continue;
+            uint8_t* dst = (uint8_t*)(uintptr_t)lane_u64(d, s2);
+            const uint8_t* src = (const uint8_t*)(uintptr_t)lane_u64(sp, s2);
+            if (large) copy_part(dst, src, bytes, t, nt);
+            else copy_part(dst, src, bytes, lane, 64);
         }
     }
 }
@@ -319,10 +338,11 @@ __device__ inline int coll_gather(bool v, const void* sendbuf, int sendcount, MP
     void* r_eff = nullptr;
     if (at_root) {
         if (v) {
-            for (int r = 0; r < cc.size; r++) {
+            for (int r = lane_id(); r < cc.size; r += 64) {
                 const uint64_t end = ((uint64_t)displs[r] + (uint64_t)recvcounts[r]) * (uint64_t)res;
                 if (end > rtotal) rtotal = end;
             }
+            rtotal = wave_max_u64(rtotal);
             publish(cc, SW_RCNT, (uint64_t)(uintptr_t)stg.in(recvcounts, sizeof(int) * (uint64_t)cc.size));
             publish(cc, SW_RDSP, (uint64_t)(uintptr_t)stg.in(displs, sizeof(int) * (uint64_t)cc.size));
         } else {
@@ -370,10 +390,11 @@ __device__ inline int coll_scatter(bool v, const void* sendbuf, int sendcount, c
         if (ses <= 0) return GPUMPI_FAIL(MPI_ERR_TYPE, "invalid datatype in scatter");
         uint64_t stotal = 0;
         if (v) {
-            for (int r = 0; r < cc.size; r++) {
+            for (int r = lane_id(); r < cc.size; r += 64) {
                 const uint64_t end = ((uint64_t)displs[r] + (uint64_t)sendcounts[r]) * (uint64_t)ses;
                 if (end > stotal) stotal = end;
             }
+            stotal = wave_max_u64(stotal);
             publish(cc, SW_SCNT, (uint64_t)(uintptr_t)stg.in(sendcounts, sizeof(int) * (uint64_t)cc.size));
             publish(cc, SW_SDSP, (uint64_t)(uintptr_t)stg.in(displs, sizeof(int) * (uint64_t)cc.size));
         } else {
@@ -421,13 +442,15 @@ __device__ inline int coll_segments(int kind, const void* sendbuf, int sendcount
     if (ses <= 0 || res <= 0) return GPUMPI_FAIL(MPI_ERR_TYPE, "invalid datatype in all-to-all/all-gather");
     if (in_place && (kind == SEG_ALLGATHER || kind == SEG_ALLGATHERV)) sendcount = 0;
     CollStage stg;
-    // Receive extent.
+    // Receive extent: the lanes read the count and displacement arrays in
+    // parallel (one round trip, not one per member).
     uint64_t rtotal = 0;
     if (kind == SEG_ALLTOALLV || kind == SEG_ALLGATHERV) {
-        for (int r = 0; r < P; r++) {
+        for (int r = lane_id(); r < P; r += 64) {
             const uint64_t end = ((uint64_t)rdispls[r] + (uint64_t)recvcounts[r]) * (uint64_t)res;
             if (end > rtotal) rtotal = end;
         }
+        rtotal = wave_max_u64(rtotal);
     } else {
         rtotal = (uint64_t)P * (uint64_t)recvcount * (uint64_t)res;
     }
@@ -438,10 +461,11 @@ __device__ inline int coll_segments(int kind, const void* sendbuf, int sendcount
     uint64_t sword_cnt = 0;
     if (kind == SEG_ALLTOALLV) {
         uint64_t stotal = 0;
-        for (int r = 0; r < P; r++) {
+        for (int r = lane_id(); r < P; r += 64) {
             const uint64_t end = ((uint64_t)sdispls[r] + (uint64_t)sendcounts[r]) * (uint64_t)ses;
             if (end > stotal) stotal = end;
         }
+        stotal = wave_max_u64(stotal);
         s_eff = stg.in(sendbuf, stotal);
         publish(cc, SW_SCNT, (uint64_t)(uintptr_t)stg.in(sendcounts, sizeof(int) * (uint64_t)P));
         publish(cc, SW_SDSP, (uint64_t)(uintptr_t)stg.in(sdispls, sizeof(int) * (uint64_t)P));

@@ -293,7 +293,7 @@ int gpumpi_default_config(gpumpi_config* cfg, int num_procs) {
     cfg->chan_slots = 8;
     cfg->eager_bytes = 448;
     cfg->helper_min_bytes = 16384;
-    cfg->xcd_map = 0;
+    cfg->xcd_map = 1;
     cfg->stack_size = 0;
     cfg->heap_bytes = (size_t)num_procs * (4u << 20);
     cfg->staging_bytes = 4u << 20;

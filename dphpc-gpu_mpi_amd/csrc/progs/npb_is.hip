@@ -97,6 +97,26 @@ __device__ inline int wave_exclusive_scan(int* v, int n, int lane) {
     return __shfl(incl, 63);
 }
 
+// out[i] = in[0] + ... + in[i-1] for i < n, by the 64 lanes of one wave:
+// lane l sums a contiguous chunk, the lanes scan their sums, then each writes
+// its chunk.  Reads only `in`, so the lanes exchange nothing through memory.
+__device__ inline void wave_exclusive_scan_to(const int* in, int* out, int n, int lane) {
+    const int chunk = (n + 63) / 64;
+    const int lo = lane * chunk, hi = min(n, lo + chunk);
+    int s = 0;
+    for (int i = lo; i < hi; i++) s += in[i];
+    int incl = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    int run = incl - s;
+    for (int i = lo; i < hi; i++) {
+        out[i] = run;
+        run += in[i];
+    }
+}
+
 }  // namespace
 
 // args: [1] class char, [2] out (int64[8]: passed, total keys, errors,
@@ -234,8 +254,10 @@ GPUMPI_PROGRAM(npb_is) {
         // is.c:758-781
         const double a0 = MPI_Wtime();
         MPI_Alltoall(send_count, 1, MPI_INT, recv_count, 1, MPI_INT, comm_work);
-        recv_displ[0] = 0;
-        for (int i = 1; i < comm_size; i++) recv_displ[i] = recv_displ[i - 1] + recv_count[i - 1];
+        // recv_displ = exclusive prefix of recv_count (is.c:766-768), by the
+        // program's wave in one pass instead of a loop of dependent loads.
+        wave_exclusive_scan_to(recv_count, recv_displ, comm_size, gpumpi::rt::lane_id());
+        gpumpi::rt::wave_sync();  // MPI_Alltoallv's lanes read entries other lanes wrote
         MPI_Alltoallv(key_buff1, send_count, send_displ, MPI_INT, key_buff2, recv_count, recv_displ, MPI_INT,
                       comm_work);
         t_a2a += MPI_Wtime() - a0;

@@ -58,6 +58,7 @@ inline void __syncthreads() { gpumpi_host_unsupported("syncthreads"); }
 template <typename T>
 inline T gpumpi_host_readfirstlane(T) { gpumpi_host_unsupported("readfirstlane"); }
 #define __builtin_amdgcn_readfirstlane(v) gpumpi_host_readfirstlane(v)
+#define __builtin_amdgcn_readlane(v, l) gpumpi_host_readfirstlane(v)
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 #define __builtin_amdgcn_s_sleep(n) ((void)0)
 #define __builtin_amdgcn_wave_barrier() ((void)0)

@@ -1,5 +1,5 @@
 // Host build of the all-to-all / all-gather data phase (coll.hpp:
-// seg_info, run_segments, copy_part) under AddressSanitizer.  The
+// seg_info, copy_part) under AddressSanitizer.  The
 // collective table is filled the way coll_segments publishes it; then every
 // member's segment copy runs for each of the 64 lanes of wave 0, so every
 // address the device would touch is checked against exactly sized buffers.
@@ -100,9 +100,18 @@ int main(int argc, char** argv) {
         c.a[0] = (uint64_t)(uintptr_t)tab.data();
         c.a[1] = es;
         c.a[2] = es;
-        for (int lane = 0; lane < 64; lane++) {
-            gpumpi_host_lane = lane;
-            run_segments(c, lane, 64);
+        // Every segment's descriptor (seg_info) and copy, lane by lane, as
+        // one wave does them (run_segments gathers the descriptors across
+        // lanes first; that exchange runs in the whole-kernel simulation).
+        for (int k = 0; k < P; k++) {
+            uint8_t* d;
+            const uint8_t* sp;
+            uint64_t n;
+            seg_info(c, k, &d, &sp, &n);
+            for (int lane = 0; lane < 64; lane++) {
+                gpumpi_host_lane = lane;
+                copy_part_lanes(d, sp, n, lane, 64);
+            }
         }
     }
     // Expected contents.

@@ -104,6 +104,7 @@ inline void __syncthreads() { ::simt::block_barrier(); }
 template <typename T>
 inline T simt_readfirstlane(T v) { return simt_from<T>(::simt::wave_op(::simt::OP_FIRST, simt_bits(v), 0)); }
 #define __builtin_amdgcn_readfirstlane(v) simt_readfirstlane(v)
+#define __builtin_amdgcn_readlane(v, l) __shfl((v), (l))
 #define __builtin_amdgcn_fence(order, scope) __atomic_signal_fence(order)
 #define __builtin_amdgcn_s_sleep(n) ::simt::yield()
 #define __builtin_amdgcn_wave_barrier() ((void)::simt::wave_op(::simt::OP_SYNC, 0, 0))
