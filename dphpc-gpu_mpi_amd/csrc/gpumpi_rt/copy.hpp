@@ -20,7 +20,7 @@ __device__ __forceinline__ bool is_global_ptr(const void* p) {
     return !is_private_ptr(p) && !is_shared_ptr(p);
 }
 
-constexpr int kCopyUnroll = 4;
+constexpr int kCopyUnroll = 8;
 
 // Global-memory (HBM) pointers: loads and stores through them are global_*
 // instructions, which count on vmcnt alone and cannot alias the stack.  (The
@@ -43,16 +43,28 @@ __device__ __forceinline__ u32x4 realign16(const u32x4 lo, const u32x4 hi, uint3
     return o;
 }
 
-// d4[i] = s4[i] for i < nvec, worker t of nt taking i = t, t + nt, ...; four
-// loads in flight per lane before the four stores.  The four values are
-// named scalars, not an array: an array here was kept in scratch (three
-// dead 16-byte stack stores per iteration) and its stores forced a wait
-// after every load.
+// d4[i] = s4[i] for i < nvec, worker t of nt taking i = t, t + nt, ...;
+// kCopyUnroll loads in flight per lane before the matching stores (a wave
+// moves 64 * 16 * kCopyUnroll bytes per round trip).  The values are named
+// scalars, not an array: an array here was kept in scratch (dead 16-byte
+// stack stores per iteration) and its stores forced a wait after every load.
 template <typename DP, typename SP>
 __device__ __forceinline__ void copy_aligned(DP d4, SP s4, uint64_t nvec, int t, int nt) {
-    static_assert(kCopyUnroll == 4, "copy_aligned moves four vectors per step");
+    static_assert(kCopyUnroll == 8, "copy_aligned moves eight vectors per step");
     uint64_t i = (uint64_t)t;
     const uint64_t st = (uint64_t)nt;
+    for (; i + 7 * st < nvec; i += 8 * st) {
+        const u32x4 v0 = s4[i], v1 = s4[i + st], v2 = s4[i + 2 * st], v3 = s4[i + 3 * st];
+        const u32x4 v4 = s4[i + 4 * st], v5 = s4[i + 5 * st], v6 = s4[i + 6 * st], v7 = s4[i + 7 * st];
+        d4[i] = v0;
+        d4[i + st] = v1;
+        d4[i + 2 * st] = v2;
+        d4[i + 3 * st] = v3;
+        d4[i + 4 * st] = v4;
+        d4[i + 5 * st] = v5;
+        d4[i + 6 * st] = v6;
+        d4[i + 7 * st] = v7;
+    }
     for (; i + 3 * st < nvec; i += 4 * st) {
         const u32x4 v0 = s4[i], v1 = s4[i + st], v2 = s4[i + 2 * st], v3 = s4[i + 3 * st];
         d4[i] = v0;
@@ -70,6 +82,16 @@ template <int Q>
 __device__ inline void copy_realigned(gvec4* d4, const gvec4* a4, uint64_t nvec, uint32_t r, int t, int nt) {
     uint64_t i = (uint64_t)t;
     const uint64_t st = (uint64_t)nt;
+    for (; i + 7 * st < nvec; i += 8 * st) {
+        u32x4 l[8], h[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            l[u] = a4[i + u * st];
+            h[u] = a4[i + u * st + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) d4[i + u * st] = realign16<Q>(l[u], h[u], r);
+    }
     for (; i + 3 * st < nvec; i += 4 * st) {
         const u32x4 l0 = a4[i], h0 = a4[i + 1];
         const u32x4 l1 = a4[i + st], h1 = a4[i + st + 1];

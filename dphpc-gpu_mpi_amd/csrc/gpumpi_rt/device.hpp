@@ -238,8 +238,14 @@ __device__ __forceinline__ int my_rank() { return ctx()->rank; }
 // ---------------------------------------------------------------------------
 __device__ inline bool aborted() {
     int v = 0;
-    if (elected()) v = ld_relaxed_system(world()->abort_word);
+    if (elected()) v = (int)ld_relaxed_agent(world()->dev_abort) | ld_relaxed_system(world()->abort_word);
     return uniform_i32(v) != 0;
+}
+// The device abort flag alone (a rank's MPI error), without the host word.
+__device__ inline bool aborted_on_device() {
+    uint32_t v = 0;
+    if (elected()) v = ld_relaxed_agent(world()->dev_abort);
+    return uniform_u32(v) != 0;
 }
 
 __device__ inline void copy_msg(char* dst, const char* msg, int cap) {
@@ -263,6 +269,7 @@ __device__ inline __attribute__((noinline)) int raise_error(int code, int line, 
             copy_msg(e->msg, msg, (int)sizeof(e->msg));
         }
         __hip_atomic_fetch_add(&e->count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(world()->dev_abort, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(world()->abort_word, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     return code;

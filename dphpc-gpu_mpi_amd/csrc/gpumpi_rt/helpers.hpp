@@ -143,9 +143,12 @@ __device__ inline void helpers_exit() {
 // gpu_libs/gpu_libc/cuda_mpi.cuh:519-521 and works on every communicator
 // (the reference's MPI_Barrier trapped on non-WORLD, mpi.cu:324-331).
 // ---------------------------------------------------------------------------
-// Returns false when the wait ended because the launch was aborted (an MPI
-// error on some rank, or the host watchdog): the members have then not all
-// arrived, so a collective must not read their table entries and returns.
+// Returns false when the launch was aborted (an MPI error on some rank, or
+// the host watchdog): either the wait gave up, or it completed but a rank
+// raised an error first -- a rank that errs and goes on to a later barrier
+// (MPI_Finalize, say) adds an arrival that completes its peers' barrier of
+// an earlier collective.  Either way the members' table entries are not
+// all published, so a collective must not read them and returns.
 __device__ inline bool comm_barrier(CommShared* c, int size) {
     if (size <= 1) return true;
     uint64_t old = 0;
@@ -162,7 +165,7 @@ __device__ inline bool comm_barrier(CommShared* c, int size) {
         }
     }
     fence_acquire_agent();
-    return ok;
+    return ok && !aborted_on_device();
 }
 
 }  // namespace rt

@@ -135,6 +135,9 @@ struct World {
     // Collective member tables (two words per member) when they do not fit
     // in LDS beside the other per-peer arrays: [P * 2P], rank-private rows.
     uint64_t* members_hbm;
+    // Device copy of the abort flag: set (release) by a rank that raises an
+    // MPI error, read by every barrier after it completes.
+    uint32_t* dev_abort;
 };
 
 // Per-rank save block: stamp, validity magic, then RxChan[P] and TxChan[P]

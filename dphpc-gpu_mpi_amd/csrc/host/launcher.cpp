@@ -69,6 +69,7 @@ struct WorldAlloc {
     uint8_t* world_tables = nullptr;  // world_of, rank_of_world, slots of WORLD; self tables
     uint8_t* saves = nullptr;         // per-rank channel cursors between launches
     uint64_t* members_hbm = nullptr;  // [P * 2P] collective member tables (large worlds)
+    uint32_t* dev_abort = nullptr;    // device abort flag
     uint64_t save_bytes = 0;
     bool need_reset = true;           // next launch starts the channels from zero
     hipStream_t stream = nullptr;
@@ -78,7 +79,7 @@ struct WorldAlloc {
         auto f = [](void* p) { if (p) (void)hipFree(p); };
         f(slots); f(chan_head); f(done_words); f(staging); f(arena); f(arena_top);
         f(comm_world); f(comm_self); f(comm_pool); f(comm_pool_top); f(next_ctx); f(rank_ret);
-        f(rank_stats); f(dworld); f(world_tables); f(saves); f(members_hbm);
+        f(rank_stats); f(dworld); f(world_tables); f(saves); f(members_hbm); f(dev_abort);
         if (abort_word) (void)hipHostFree(abort_word);
         if (err) (void)hipHostFree(err);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -126,6 +127,7 @@ int alloc_world(const Config& cfg) {
     W.save_bytes = save_block_bytes(cfg.nranks);
     HIPOK(hipMalloc(&W.saves, P * W.save_bytes));
     HIPOK(hipMalloc(&W.members_hbm, P * 2 * P * sizeof(uint64_t)));
+    HIPOK(hipMalloc(&W.dev_abort, 256));
     W.need_reset = true;
     HIPOK(hipHostMalloc((void**)&W.abort_word, 64, hipHostMallocCoherent));
     HIPOK(hipHostMalloc((void**)&W.err, sizeof(ErrorRecord), hipHostMallocCoherent));
@@ -188,6 +190,7 @@ int init_world(const Config& cfg, void* user_args) {
     W.abort_word[0] = 0;
     W.abort_word[1] = 0;  // set by a rank that exits with undelivered messages
     std::memset((void*)W.err, 0, sizeof(ErrorRecord));
+    HIPOK(hipMemsetAsync(W.dev_abort, 0, 256, s));
 
     World hw{};
     hw.cfg = cfg;
@@ -209,6 +212,7 @@ int init_world(const Config& cfg, void* user_args) {
     hw.user_args = user_args;
     hw.saves = W.saves;
     hw.members_hbm = W.members_hbm;
+    hw.dev_abort = W.dev_abort;
     hw.save_bytes = W.save_bytes;
     HIPOK(hipMemcpyAsync(W.dworld, &hw, sizeof hw, hipMemcpyHostToDevice, s));
     HIPOK(hipStreamSynchronize(s));
@@ -538,6 +542,8 @@ int gpumpi_run_graph(const gpumpi_config* cfg_in, const void* kernel, int argc, 
         W.abort_word[0] = 0;
         W.abort_word[1] = 0;
         std::memset((void*)W.err, 0, sizeof(ErrorRecord));
+        // Between replays, outside the captured graph.
+        HIPOK(hipMemsetAsync(W.dev_abort, 0, 256, W.stream));
         HIPOK(hipEventRecord(W.ev0, W.stream));
         HIPOK(hipGraphLaunch(exec, W.stream));
         HIPOK(hipEventRecord(W.ev1, W.stream));

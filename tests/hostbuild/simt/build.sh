@@ -1,25 +1,36 @@
 #!/bin/bash
-# Builds the CPU simulation of the runtime + launcher + rank programs with a
-# scenario driver: build.sh OUT DRIVER.cpp [extra flags]
+# Builds the CPU simulation of the device runtime, the host launcher and rank
+# programs, and links one executable per scenario driver:
+#   build.sh OUTDIR DRIVER.cpp [DRIVER.cpp ...]   -> OUTDIR/<driver name>
+# SIMT_PROGS (optional): space-separated program sources (progs/) to build
+# instead of every progs/*.hip file.
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 ROOT=$(cd "$HERE/../../.." && pwd)
-OUT=$1; DRIVER=$2; shift 2
+OUT=$1; shift
 CXX=${CXX:-/opt/rocm/llvm/bin/clang++}
-FLAGS="-std=c++17 -g -O1 -fno-omit-frame-pointer -ffp-contract=off -fsanitize=address,undefined -fno-sanitize=alignment,function -mllvm -asan-globals=0 -fsanitize-address-use-after-return=never -I $HERE -I $ROOT/include -I $ROOT/dphpc-gpu_mpi_amd/csrc $*"
-mkdir -p "$(dirname "$OUT")/obj"
-OBJ="$(dirname "$OUT")/obj"
-srcs="$ROOT/dphpc-gpu_mpi_amd/csrc/host/launcher.cpp $ROOT/dphpc-gpu_mpi_amd/csrc/progs/registry.cpp $HERE/simt.cpp $DRIVER"
-for p in $ROOT/dphpc-gpu_mpi_amd/csrc/progs/*.hip; do srcs="$srcs $p"; done
-objs=""
+FLAGS="-std=c++17 -g -O1 -fno-omit-frame-pointer -ffp-contract=off -fsanitize=address,undefined -fno-sanitize=alignment,function -mllvm -asan-globals=0 -fsanitize-address-use-after-return=never -I $HERE -I $ROOT/include -I $ROOT/dphpc-gpu_mpi_amd/csrc"
+COV="-fsanitize-coverage=inline-8bit-counters,trace-loads,trace-stores"
+OBJ="$OUT/obj"
+mkdir -p "$OBJ"
+common="$ROOT/dphpc-gpu_mpi_amd/csrc/host/launcher.cpp $ROOT/dphpc-gpu_mpi_amd/csrc/progs/registry.cpp"
+if [ -n "$SIMT_PROGS" ]; then
+  for p in $SIMT_PROGS; do common="$common $ROOT/dphpc-gpu_mpi_amd/csrc/progs/$p"; done
+else
+  for p in $ROOT/dphpc-gpu_mpi_amd/csrc/progs/*.hip; do common="$common $p"; done
+fi
 pids=""
-for s in $srcs; do
+objs=""
+for s in $common "$@"; do
   o="$OBJ/$(basename $s).o"
-  objs="$objs $o"
-  cov="-fsanitize-coverage=inline-8bit-counters,trace-loads,trace-stores"
-  case "$s" in *simt.cpp) cov="";; esac
-  $CXX $FLAGS $cov -x c++ -c "$s" -o "$o" &
+  $CXX $FLAGS $COV -x c++ -c "$s" -o "$o" &
   pids="$pids $!"
 done
+# the scheduler itself is not instrumented (its own accesses are not lanes')
+$CXX $FLAGS -c "$HERE/simt.cpp" -o "$OBJ/simt.cpp.o" &
+pids="$pids $!"
 for p in $pids; do wait $p; done
-$CXX $FLAGS $objs -o "$OUT"
+for s in $common; do objs="$objs $OBJ/$(basename $s).o"; done
+for d in "$@"; do
+  $CXX $FLAGS $objs "$OBJ/simt.cpp.o" "$OBJ/$(basename $d).o" -o "$OUT/$(basename $d .cpp)"
+done

@@ -2,6 +2,10 @@
 // with the shapes of the GPU tests, checked against the expected layouts.
 //   run_coll alltoallv COUNTS_FILE THREADS     (COUNTS_FILE: P, then P*P counts)
 //   run_coll blocks KIND THREADS               (P = 6, block 5, root 2)
+//   run_coll abort P THREADS                   (rank 1 aborts; the others keep
+//                                               calling collectives)
+#include "progs/registry.hpp"
+
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -13,7 +17,30 @@
 
 extern "C" const void* gpumpi_prog_lookup(const char* name);
 
-static int launch(const char* prog, int P, int threads, std::vector<uint64_t>& words) {
+// Rank 1 raises an MPI error; every other rank runs Allreduce, Bcast and
+// Alltoall.  The launch must end with that error and no rank may read a
+// collective table entry its peers never published.
+GPUMPI_PROGRAM(sim_abort_collectives) {
+    MPI_Init(&argc, &argv);
+    int rank, P;
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    MPI_Comm_size(MPI_COMM_WORLD, &P);
+    int* buf = (int*)gpumpi_malloc(sizeof(int) * 4 * (uint64_t)P);
+    for (int i = 0; i < 4 * P; i++) buf[i] = rank + i;
+    if (rank == 1) {
+        MPI_Abort(MPI_COMM_WORLD, 7);
+    } else {
+        for (int it = 0; it < 3; it++) {
+            MPI_Allreduce(MPI_IN_PLACE, buf, P, MPI_INT, MPI_SUM, MPI_COMM_WORLD);
+            MPI_Bcast(buf, P, MPI_INT, 0, MPI_COMM_WORLD);
+            MPI_Alltoall(buf, 1, MPI_INT, buf + 2 * P, 1, MPI_INT, MPI_COMM_WORLD);
+        }
+    }
+    MPI_Finalize();
+    return 0;
+}
+
+static int launch(const char* prog, int P, int threads, std::vector<uint64_t>& words, int* mpi_error = nullptr) {
     gpumpi_config cfg;
     gpumpi_default_config(&cfg, P);
     cfg.threads_per_rank = threads;
@@ -27,6 +54,7 @@ static int launch(const char* prog, int P, int threads, std::vector<uint64_t>& w
                 res.mpi_error, res.mpi_error_rank, res.mpi_error_line, res.mpi_error_msg);
     if (rc) std::printf("error: %s\n", gpumpi_last_error());
     std::fflush(stdout);
+    if (mpi_error) *mpi_error = res.mpi_error;
     return rc;
 }
 
@@ -116,9 +144,18 @@ static int blocks(int kind, int threads) {
     return bad ? 1 : 0;
 }
 
+static int abort_case(int P, int threads) {
+    std::vector<uint64_t> words(64, 0);
+    int err = 0;
+    const int rc = launch("sim_abort_collectives", P, threads, words, &err);
+    std::printf("abort case: rc=%d mpi_error=%d (expect rc=%d, mpi_error=7)\n", rc, err, GPUMPI_EDEVICE);
+    return (rc == GPUMPI_EDEVICE && err == 7) ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 4 && !std::strcmp(argv[1], "abort")) return abort_case(std::atoi(argv[2]), std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "alltoallv")) return alltoallv(argv[2], std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "blocks")) return blocks(std::atoi(argv[2]), std::atoi(argv[3]));
-    std::fprintf(stderr, "usage: run_coll alltoallv COUNTS THREADS | blocks KIND THREADS\n");
+    std::fprintf(stderr, "usage: run_coll alltoallv COUNTS THREADS | blocks KIND THREADS | abort P THREADS\n");
     return 2;
 }
