@@ -123,14 +123,14 @@ struct FoldArgs {
 // Members are folded in batches of kFoldBatch: the batch's loads are all
 // issued before its values are folded in rank order, so a lane keeps
 // kFoldBatch * U vectors in flight instead of one.
-constexpr int kFoldBatch = 8;
+constexpr int kFoldBatch = 4;
 
 template <typename T, typename F>
 __device__ inline void fold_slice(const FoldArgs& A, int t, int nt) {
     F f;
     const int P = A.nmembers;
     constexpr int V = sizeof(T) >= 16 ? 1 : (int)(16 / sizeof(T));
-    constexpr int U = 2;
+    constexpr int U = 4;
     constexpr int B = kFoldBatch;
     uint64_t lo = A.lo;
     const uint64_t hi = A.hi;

@@ -176,7 +176,7 @@ __device__ __forceinline__ void wire_group(int fmt, const uint8_t* w, uint64_t n
 // members' loads in flight, and writes the 16 sums to every member's
 // output.  The last, partial group goes element by element.
 __device__ inline void wire_fold_part(const WireFoldArgs& A, int t, int nt) {
-    constexpr int B = kFoldBatch / 2;
+    constexpr int B = 4;  // members per batch (16 values each in flight)
     const int P = A.nmembers;
     const uint64_t full_groups = A.n / kWireGroup;
     for (uint64_t g = A.g0 + (uint64_t)t; g < A.g1; g += (uint64_t)nt) {
