@@ -88,11 +88,11 @@ def copy_calibration(nbytes):
     return 0
 
 
-def side_measurements(gm, np):
+def side_measurements(gm, np, threads):
     """Same-run figures for the other headline rows: exact FP32 Allreduce
     (256 ranks x 16 MiB, algorithmic 2*P*N*4 bytes) and NPB IS class B on
     128 ranks (keys/s, and its Alltoallv key exchange, 2 * 2^25 * 4 bytes
-    per iteration)."""
+    per iteration), with `threads` per rank as the ring."""
     out = {}
     P, n, iters = 256, 1 << 22, 10
     ib = gm.DeviceBuffer(4 * n * P)
@@ -100,14 +100,14 @@ def side_measurements(gm, np):
     ib.zero()
     times = gm.DeviceBuffer(8 * P)
     gm.run("coll_reduce", P, [0, gm.DT["MPI_FLOAT"], gm.OP["MPI_SUM"], n, ib, ob, 4 * n, -1, 0, 1, iters, times, 0, 0],
-           threads_per_rank=256, timeout_s=300.0)
+           threads_per_rank=threads, timeout_s=300.0)
     t = float(times.download(np.float64).max()) / iters
     out["allreduce_f32_256x16MiB_gbps"] = round(2.0 * P * n * 4 / t / 1e9, 1)
     del ib, ob
     res = gm.DeviceBuffer(8 * 8)
     res.zero()
     tout = gm.DeviceBuffer(8 * 4)
-    gm.run("npb_is", 128, [0, ord("B"), res, tout, 0, 10], threads_per_rank=256, user_lds_bytes=4096,
+    gm.run("npb_is", 128, [0, ord("B"), res, tout, 0, 10], threads_per_rank=threads, user_lds_bytes=4096,
            heap_bytes=128 * (32 << 20), max_requests=64, timeout_s=120.0)
     r = res.download(np.int64)
     tt = tout.download(np.float64)
@@ -124,7 +124,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--ranks", type=int, default=256)
     ap.add_argument("--bytes", type=int, default=64 << 20)
-    ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--threads", type=int, default=512,
+                    help="threads per rank workgroup (512: eight waves, one rank per CU at 256 ranks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the Allreduce / NPB IS side measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -169,14 +170,17 @@ def main(argv=None):
     wtime_gbps = bytes_per_step * args.steps / float(t.max()) / 1e9
     bad = gm.verify_payload(rb, stride, S, SEED, -1, P, P)
 
-    # in-GPU ping-pong latency, 8 B, pair (0, 1) inside the same 256-rank world
+    # in-GPU ping-pong latency, 8 B, pair (0, 1) inside the same 256-rank world,
+    # with four waves per rank (idle helper waves poll LDS beside wave 0, so
+    # fewer of them leave the small-message path less contended)
     ppb = gm.DeviceBuffer(4096 * P)
     sz = gm.DeviceBuffer.from_array(np.array([8], dtype=np.uint64))
     it = gm.DeviceBuffer.from_array(np.array([1000], dtype=np.int64))
     wm = gm.DeviceBuffer.from_array(np.array([100], dtype=np.int64))
     pt = gm.DeviceBuffer(8 * P)
+    lat_threads = 256
     gm.run("p2p_pingpong_sweep", P, [0, sz, 1, it, wm, ppb, ppb, 4096, pt, 1],
-           threads_per_rank=args.threads, timeout_s=120.0)
+           threads_per_rank=lat_threads, timeout_s=120.0)
     rtt_us = float(pt.download(np.float64)[:2].max()) / 1000 * 1e6
 
     line = {
@@ -201,11 +205,12 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "p2p_latency_us_8B_one_way": round(rtt_us / 2, 3),
         "p2p_rtt_us_8B": round(rtt_us, 3),
+        "p2p_latency_threads_per_rank": lat_threads,
         "ring_gbps_mpi_wtime": round(wtime_gbps, 2),
         "payload_mismatches": bad,
     }
     if not args.no_extras:
-        line.update(side_measurements(gm, np))
+        line.update(side_measurements(gm, np, args.threads))
     print(json.dumps(line), flush=True)
     del sb, rb, ppb
     gm.release_world()
