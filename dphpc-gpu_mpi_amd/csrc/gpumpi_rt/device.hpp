@@ -288,6 +288,16 @@ struct Spinner {
         __builtin_amdgcn_s_sleep(1);
         return true;
     }
+    // Same, without the sleep: for a receive's own poll loop, where the
+    // poll's memory round trip already paces it and a sleep only adds to
+    // the time until an arrived message is seen.
+    __device__ __forceinline__ bool keep_polling() {
+        if (++n >= ctx()->spin_period) {
+            n = 0;
+            if (aborted()) return false;
+        }
+        return true;
+    }
 };
 
 // MPI_Wtime clock: s_memrealtime, a constant-rate counter (wall_hz).

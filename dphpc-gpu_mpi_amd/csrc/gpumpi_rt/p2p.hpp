@@ -711,7 +711,7 @@ __device__ inline __attribute__((noinline)) int wait_req(int idx, MPI_Status* st
                 break;
             }
             if ((++k & 15) == 0) progress();
-            if (!sp.keep_going()) break;
+            if (!sp.keep_polling()) break;
         }
     }
     while (q->state != RS_COMPLETE) {
@@ -747,6 +747,7 @@ __device__ inline bool send_direct(const void* buf, uint64_t n, int dst, int tag
 // other requests) runs every 16th poll.
 __device__ inline int recv_direct(void* buf, uint64_t cap, int src, int tag, int ctxid, MPI_Comm comm,
                                   MPI_Status* st) {
+    RankLds* L = ctx();
     Spinner sp;
     int k = 0;
     for (;;) {
@@ -765,8 +766,11 @@ __device__ inline int recv_direct(void* buf, uint64_t cap, int src, int tag, int
             if (t.err != MPI_SUCCESS) GPUMPI_FAIL(t.err, "message truncated: receive buffer too small");
             return t.err;
         }
-        if ((++k & 15) == 0) progress();
-        if (!sp.keep_going()) return MPI_ERR_OTHER;
+        // Other requests of this rank (queued sends, matched rendezvous
+        // copies) progress every 16th poll; with none live there is nothing
+        // to do.
+        if ((++k & 15) == 0 && L->ring_tail != L->ring_head) progress();
+        if (!sp.keep_polling()) return MPI_ERR_OTHER;
     }
 }
 
