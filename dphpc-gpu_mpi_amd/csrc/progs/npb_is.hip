@@ -117,6 +117,15 @@ __device__ inline void wave_exclusive_scan_to(const int* in, int* out, int n, in
     }
 }
 
+__device__ inline long wave_sum_long(long v) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)(uint64_t)v, m);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)((uint64_t)v >> 32), m);
+        v += (long)(((uint64_t)hi << 32) | lo);
+    }
+    return v;
+}
+
 }  // namespace
 
 // args: [1] class char, [2] out (int64[8]: passed, total keys, errors,
@@ -232,17 +241,54 @@ GPUMPI_PROGRAM(npb_is) {
             int local_acc = 0, j = 0;
             send_displ[0] = 0;
             ptr1[0] = 0;
-            for (int i = 0; i < NB; i++) {
-                bucket_sum_accumulator += bucket_totals[i];
-                local_acc += bucket_size[i];
-                if (bucket_sum_accumulator >= (long)(j + 1) * num_keys) {
-                    send_count[j] = local_acc;
-                    if (j != 0) {
-                        send_displ[j] = send_displ[j - 1] + send_count[j - 1];
-                        ptr1[j] = ptr2[j - 1] + 1;
+            if (NB == 64 * 16) {
+                // The same sequential walk over the buckets, with both bucket
+                // arrays loaded once into registers (16 per lane) and read
+                // back lane by lane (v_readlane), not one global load per
+                // bucket; the previous rank's displacement, count and last
+                // bucket stay in registers too.
+                const int lane = gpumpi::rt::lane_id();
+                int bt[16], bs[16];
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    bt[r] = bucket_totals[lane * 16 + r];
+                    bs[r] = bucket_size[lane * 16 + r];
+                }
+                int prev_displ = 0, prev_count = 0, prev_ptr2 = 0;
+                for (int l = 0; l < 64; l++) {
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        bucket_sum_accumulator += __builtin_amdgcn_readlane(bt[r], l);
+                        local_acc += __builtin_amdgcn_readlane(bs[r], l);
+                        if (bucket_sum_accumulator >= (long)(j + 1) * num_keys) {
+                            const int i = l * 16 + r;
+                            send_count[j] = local_acc;
+                            if (j != 0) {
+                                prev_displ += prev_count;
+                                send_displ[j] = prev_displ;
+                                ptr1[j] = prev_ptr2 + 1;
+                            }
+                            ptr2[j] = i;
+                            prev_ptr2 = i;
+                            prev_count = local_acc;
+                            j++;
+                            local_acc = 0;
+                        }
                     }
-                    ptr2[j++] = i;
-                    local_acc = 0;
+                }
+            } else {
+                for (int i = 0; i < NB; i++) {
+                    bucket_sum_accumulator += bucket_totals[i];
+                    local_acc += bucket_size[i];
+                    if (bucket_sum_accumulator >= (long)(j + 1) * num_keys) {
+                        send_count[j] = local_acc;
+                        if (j != 0) {
+                            send_displ[j] = send_displ[j - 1] + send_count[j - 1];
+                            ptr1[j] = ptr2[j - 1] + 1;
+                        }
+                        ptr2[j++] = i;
+                        local_acc = 0;
+                    }
                 }
             }
             while (j < comm_size) {
@@ -265,12 +311,15 @@ GPUMPI_PROGRAM(npb_is) {
         const int min_key_val = ptr1[my_rank] << shift;
         const int max_key_val = ((ptr2[my_rank] + 1) << shift) - 1;
         const int range = max_key_val - min_key_val + 1;
+        // keys of the lower ranks' buckets (m) and of mine (nloc): the same
+        // sums, with the lanes taking ranks / buckets in parallel
         long m = 0;
-        for (int k = 0; k < my_rank; k++)
+        for (int k = gpumpi::rt::lane_id(); k < my_rank; k += 64)
             for (int i = ptr1[k]; i <= ptr2[k]; i++) m += bucket_totals[i];
-        int j = 0;
-        for (int i = ptr1[my_rank]; i <= ptr2[my_rank]; i++) j += bucket_totals[i];
-        const int nloc = j;
+        m = wave_sum_long(m);
+        long jl = 0;
+        for (int i = ptr1[my_rank] + gpumpi::rt::lane_id(); i <= ptr2[my_rank]; i += 64) jl += bucket_totals[i];
+        const int nloc = (int)wave_sum_long(jl);
         gpumpi_parallel([=] __device__(int t, int nt) {
             for (int i = t; i < range; i += nt) counts[i] = 0;
         });
