@@ -107,7 +107,7 @@ def side_measurements(gm, np, threads):
     res = gm.DeviceBuffer(8 * 8)
     res.zero()
     tout = gm.DeviceBuffer(8 * 4)
-    gm.run("npb_is", 128, [0, ord("B"), res, tout, 0, 10], threads_per_rank=threads, user_lds_bytes=4096,
+    gm.run("npb_is", 128, [0, ord("B"), res, tout, 0, 10], threads_per_rank=threads, user_lds_bytes=4096 + 98304,
            heap_bytes=128 * (32 << 20), max_requests=64, timeout_s=120.0)
     r = res.download(np.int64)
     tt = tout.download(np.float64)

@@ -166,7 +166,7 @@ GPUMPI_PROGRAM(npb_is) {
     int* send_displ = recv_count + comm_size;
     int* recv_displ = send_displ + comm_size;
     int* scratch = (int*)gpumpi_malloc(sizeof(int) * 2048);  // per-thread partials
-    int* hist = gpumpi_user_lds<int>();                        // NB bucket counters
+    int* hist = gpumpi_user_lds<int>();  // NB bucket counters, then (if room) the ranking counts
 
     // create_seq (is.c:535-553) for my slice, in parallel: thread t jumps
     // the generator to its first key.
@@ -320,12 +320,29 @@ GPUMPI_PROGRAM(npb_is) {
         long jl = 0;
         for (int i = ptr1[my_rank] + gpumpi::rt::lane_id(); i <= ptr2[my_rank]; i += 64) jl += bucket_totals[i];
         const int nloc = (int)wave_sum_long(jl);
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            for (int i = t; i < range; i += nt) counts[i] = 0;
-        });
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            for (int i = t; i < nloc; i += nt) atomicAdd(&counts[key_buff2[i] - min_key_val], 1);
-        });
+        // Key counts over my key range: in LDS when the range fits beside the
+        // bucket counters (the user LDS the launch reserved), then copied to
+        // `counts`; else straight in global memory.
+        int* lcounts = hist + NB;
+        const int lcap = (int)((gpumpi::rt::ctx()->lds_bytes - gpumpi::rt::ctx()->off_user) / sizeof(int)) - NB;
+        if (range <= lcap) {
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < range; i += nt) lcounts[i] = 0;
+            });
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < nloc; i += nt) atomicAdd(&lcounts[key_buff2[i] - min_key_val], 1);
+            });
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < range; i += nt) counts[i] = lcounts[i];
+            });
+        } else {
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < range; i += nt) counts[i] = 0;
+            });
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < nloc; i += nt) atomicAdd(&counts[key_buff2[i] - min_key_val], 1);
+            });
+        }
         // inclusive prefix over the key range (is.c:829-834): per-thread
         // chunk sums, a wave scan of the sums, then the chunk scans.
         gpumpi_parallel([=] __device__(int t, int nt) {

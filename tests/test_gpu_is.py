@@ -10,13 +10,18 @@ pytestmark = pytest.mark.gpu
 TOTAL = {"S": 1 << 16, "W": 1 << 20, "A": 1 << 23, "B": 1 << 25}
 
 
+# Classes A and B get room in LDS for the ranking counts (the program's LDS
+# path); S and W keep only the bucket counters (its global-memory path).
+USER_LDS = {"S": 4096, "W": 4096, "A": 4096 + 98304, "B": 4096 + 98304}
+
+
 def run_is(gm, cls, P, want_keys=True, threads=256):
     out = gm.DeviceBuffer(8 * 8)
     out.zero()
     tout = gm.DeviceBuffer(8 * 4)
     keys = gm.DeviceBuffer(4 * TOTAL[cls]) if want_keys else None
     res = gm.run("npb_is", P, [0, ord(cls), out, tout, keys, 10], threads_per_rank=threads,
-                 user_lds_bytes=4096, heap_bytes=P * (32 << 20) if cls == "B" else P * (64 << 20) // max(1, P // 8),
+                 user_lds_bytes=USER_LDS[cls], heap_bytes=P * (32 << 20) if cls == "B" else P * (64 << 20) // max(1, P // 8),
                  max_requests=64, timeout_s=120.0)
     assert res.status == 0, res.describe()
     o = out.download(np.int64)
