@@ -112,13 +112,6 @@ __device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_
     }
 }
 
-// Segment k's descriptor from lane `from` of the wave (wave-uniform result).
-__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int from) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, from);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), from);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // The segments of a multi-segment copy are visited in the order
 // j = 1..P (peer k = (me + j) mod P, spreading the sources).  A wave first
 // loads the descriptors of up to 64 of its segments at once, one lane per

@@ -43,6 +43,12 @@ __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_read
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+// Lane `from`'s value of v (`from` wave-uniform; wave-uniform result).
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int from) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, from);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), from);
+    return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {

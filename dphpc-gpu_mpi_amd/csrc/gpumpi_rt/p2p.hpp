@@ -469,6 +469,28 @@ __device__ inline bool scan_channel(int src, int ctxid, int tag, Found* f) {
     const int Q = L->Q;
     const RxChan rc = rxc()[src];
     const int j = lane_id();
+    // The oldest unconsumed slot (the head) must have arrived before any
+    // slot can match (arrived prefix, below), so a poll first reads only the
+    // head slot's first 16 words -- one 128-byte line, not the Q envelopes
+    // and a 512-byte chunk.  When the head holds the match (the in-order
+    // case) that is the whole poll; words past these are loaded by
+    // slot_take's validation.  A head for another context or tag falls
+    // through to the scan of the whole window.
+    {
+        const uint64_t* head_slot = (const uint64_t*)slot_at(L->rank, src, rc.head);
+        const int hw = L->slot_bytes / 8 < 16 ? L->slot_bytes / 8 : 16;
+        const uint64_t hv = j < hw ? ld_relaxed_agent(&head_slot[j]) : 0;
+        const uint64_t env0 = uniform_u64(lane_u64(hv, 0));
+        if (env_seq(env0) != rc.head + 1u) return false;
+        if (env_ctx(env0) == ctxid && (tag == MPI_ANY_TAG || env_tag(env0) == tag)) {
+            f->src = src;
+            f->seq = rc.head;
+            f->env = env0;
+            f->has_pre = true;
+            f->pre = hv;
+            return true;
+        }
+    }
     bool ok = false, seen = false;
     uint64_t env = 0;
     // Alongside the envelopes, the first chunk of the oldest slot: when that
