@@ -66,7 +66,10 @@ __device__ inline void stage_back(void* user, const void* staged, uint64_t bytes
 // Commands run by every wave of a rank
 // ---------------------------------------------------------------------------
 // Segment k of a multi-segment copy into this member's receive buffer.
-__device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_t** src, uint64_t* bytes) {
+// The copy is min(send block, receive block); a send block larger than the
+// receive block sets *trunc (MPI_ERR_TRUNCATE).
+__device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_t** src, uint64_t* bytes,
+                                bool* trunc) {
     const CollSlot* tab = (const CollSlot*)(uintptr_t)c.a[0];
     const int me = c.i1;
     const uint64_t ses = c.a[1], res = c.a[2];
@@ -82,6 +85,7 @@ __device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_
         const int* rdsp = (const int*)(uintptr_t)mine.w[SW_RDSP];
         uint64_t sb = (uint64_t)scnt[me] * ses, rb = (uint64_t)rcnt[k] * res;
         *bytes = sb < rb ? sb : rb;
+        *trunc = sb > rb;
         *src = sbuf + (int64_t)sdsp[me] * (int64_t)ses;
         *dst = rbuf + (int64_t)rdsp[k] * (int64_t)res;
         break;
@@ -89,6 +93,7 @@ __device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_
     case SEG_ALLTOALL: {
         const uint64_t sblk = peer.w[SW_SCNT] * ses, rblk = mine.w[SW_RCNT] * res;
         *bytes = sblk < rblk ? sblk : rblk;
+        *trunc = sblk > rblk;
         *src = sbuf + (uint64_t)me * sblk;
         *dst = rbuf + (uint64_t)k * rblk;
         break;
@@ -96,6 +101,7 @@ __device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_
     case SEG_ALLGATHER: {
         const uint64_t sblk = peer.w[SW_SCNT] * ses, rblk = mine.w[SW_RCNT] * res;
         *bytes = sblk < rblk ? sblk : rblk;
+        *trunc = sblk > rblk;
         *src = sbuf;
         *dst = rbuf + (uint64_t)k * rblk;
         break;
@@ -105,6 +111,7 @@ __device__ inline void seg_info(const Cmd& c, int k, uint8_t** dst, const uint8_
         const int* rdsp = (const int*)(uintptr_t)mine.w[SW_RDSP];
         const uint64_t sb = peer.w[SW_SCNT] * ses, rb = (uint64_t)rcnt[k] * res;
         *bytes = sb < rb ? sb : rb;
+        *trunc = sb > rb;
         *src = sbuf;
         *dst = rbuf + (int64_t)rdsp[k] * (int64_t)res;
         break;
@@ -130,17 +137,23 @@ __device__ inline void run_segments(const Cmd& c, int t, int nt) {
     for (int b = 0; b < count; b += 64) {
         // Lane l: descriptor of this wave's segment b + l.
         uint64_t d = 0, sp = 0, n = 0;
+        bool trunc = false;
         const int i = b + lane;
         if (i < count) {
             const int k = (me + first + i * step) % P;
             if (!(k == me && skip_self)) {
                 uint8_t* dd;
                 const uint8_t* ss;
-                seg_info(c, k, &dd, &ss, &n);
+                seg_info(c, k, &dd, &ss, &n, &trunc);
                 d = (uint64_t)(uintptr_t)dd;
                 sp = (uint64_t)(uintptr_t)ss;
             }
         }
+        // Every worker visits each descriptor once in the large mode, so only
+        // the wave that owns worker 0 reports; in the small mode each wave
+        // reports its own segments.
+        if (ballot(trunc) && (!large || t < 64))
+            GPUMPI_FAIL(MPI_ERR_TRUNCATE, "all-to-all/all-gather block larger than the receive block");
         const int nb = count - b < 64 ? count - b : 64;
         for (int s2 = 0; s2 < nb; s2++) {
             const uint64_t bytes = lane_u64(n, s2);
@@ -322,8 +335,10 @@ __device__ inline int coll_gather(bool v, const void* sendbuf, int sendcount, MP
     int err = coll_begin(comm, &cc);
     if (err) return err;
     if (root < 0 || root >= cc.size) return GPUMPI_FAIL(MPI_ERR_RANK, "invalid root in gather");
-    const int ses = type_size(sendtype);
     const bool at_root = cc.me == root;
+    if (!at_root && sendbuf == MPI_IN_PLACE)
+        return GPUMPI_FAIL(MPI_ERR_BUFFER, "MPI_IN_PLACE is valid only at the root of a gather");
+    const int ses = sendbuf == MPI_IN_PLACE ? 1 : type_size(sendtype);
     const int res = at_root ? type_size(recvtype) : 1;
     if (ses <= 0 || res <= 0) return GPUMPI_FAIL(MPI_ERR_TYPE, "invalid datatype in gather");
     CollStage stg;
@@ -452,7 +467,26 @@ __device__ inline int coll_segments(int kind, const void* sendbuf, int sendcount
     // Send extent and own contribution.
     const void* s_eff;
     uint64_t sword_cnt = 0;
-    if (kind == SEG_ALLTOALLV) {
+    // In-place all-to-all: the blocks to send are the receive buffer's own
+    // (recvcounts/rdispls, recvtype), so they travel from a copy of it on
+    // the rank heap while the peers overwrite the receive buffer; the own
+    // block is already in place.
+    uint8_t* ip_copy = nullptr;
+    if (in_place && (kind == SEG_ALLTOALLV || kind == SEG_ALLTOALL)) {
+        ip_copy = (uint8_t*)rank_malloc(rtotal ? rtotal : 1);
+        if (!ip_copy) return MPI_ERR_NO_MEM;
+        rank_copy(ip_copy, r_eff, rtotal);
+        s_eff = ip_copy;
+        if (kind == SEG_ALLTOALLV) {
+            publish(cc, SW_SCNT, (uint64_t)(uintptr_t)stg.in(recvcounts, sizeof(int) * (uint64_t)P));
+            publish(cc, SW_SDSP, (uint64_t)(uintptr_t)stg.in(rdispls, sizeof(int) * (uint64_t)P));
+            publish(cc, SW_RCNT, (uint64_t)(uintptr_t)stg.in(recvcounts, sizeof(int) * (uint64_t)P));
+            publish(cc, SW_RDSP, (uint64_t)(uintptr_t)stg.in(rdispls, sizeof(int) * (uint64_t)P));
+        } else {
+            publish(cc, SW_SCNT, (uint64_t)recvcount);
+            publish(cc, SW_RCNT, (uint64_t)recvcount);
+        }
+    } else if (kind == SEG_ALLTOALLV) {
         uint64_t stotal = 0;
         for (int r = lane_id(); r < P; r += 64) {
             const uint64_t end = ((uint64_t)sdispls[r] + (uint64_t)sendcounts[r]) * (uint64_t)ses;
@@ -503,6 +537,7 @@ __device__ inline int coll_segments(int kind, const void* sendbuf, int sendcount
     c.a[2] = (uint64_t)res;
     run_sized(c, 2 * rtotal);
     if (!comm_barrier(cc.c, cc.size)) return MPI_ERR_OTHER;
+    if (ip_copy) rank_free(ip_copy, rtotal ? rtotal : 1);
     stage_back(recvbuf, r_eff, rtotal);
     return MPI_SUCCESS;
 }

@@ -107,7 +107,9 @@ enum ReqState : uint8_t {
     RS_COMPLETE = 2,
     RS_MATCHED = 3,   // recv matched to a rendezvous message; the copy is still to do
 };
-enum ReqFlags : uint8_t { RF_SYNC = 1, RF_STAGED = 2, RF_PRIVATE_DST = 4, RF_PROC_NULL = 8 };
+// RF_DETACHED: the handle was freed by MPI_Request_free; progress releases
+// the request once it completes.
+enum ReqFlags : uint8_t { RF_SYNC = 1, RF_STAGED = 2, RF_PRIVATE_DST = 4, RF_PROC_NULL = 8, RF_DETACHED = 16 };
 
 struct Req {
     uint8_t kind;
@@ -192,6 +194,7 @@ struct RankLds {
     int pending_copies;  // matched rendezvous receives whose copy is still to do
     uint64_t bounce_top; // staging bump offset for private rendezvous sends
     int bounce_users;
+    int detached;        // requests freed by MPI_Request_free that are still active
     // offsets of the variable arrays from the LDS base
     uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user;
     uint32_t lds_bytes;

@@ -21,17 +21,15 @@ __device__ inline uint32_t wg_post(const Cmd& c) {
     __hip_atomic_store(&L->cmd_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     return seq;
 }
+// Every command is a bounded data-parallel loop that waits on no other
+// rank, so this wait always ends and never gives up, not even after an
+// abort: wave 0 must not reuse the command, the LDS member table or the
+// buffers a command names while a helper may still be reading them.
 __device__ inline void wg_wait(uint32_t seq) {
     RankLds* L = ctx();
     const uint32_t target = seq * (uint32_t)(L->waves - 1);
-    int n = 0;
-    while (__hip_atomic_load(&L->ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+    while (__hip_atomic_load(&L->ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target)
         __builtin_amdgcn_s_sleep(1);
-        if (++n >= (1 << 20)) {
-            n = 0;
-            if (aborted()) break;
-        }
-    }
 }
 
 // Run `c` on every wave of this rank (wave 0 included) and return when all
@@ -102,21 +100,17 @@ __device__ inline void rank_copy(void* dst, const void* src, uint64_t n) {
     wg_wait(seq);
 }
 
-// Body of waves 1..W-1.
+// Body of waves 1..W-1.  The helpers leave only on CMD_EXIT, which wave 0
+// posts when the rank's program returns (also after an abort: every wait
+// of wave 0 ends on the abort word), so a command posted after an abort
+// still finds its helpers.
 __device__ inline void helper_loop() {
     RankLds* L = ctx();
     uint32_t seen = 0;
-    const int period = 4096;
     for (;;) {
         uint32_t s;
-        int n = 0;
-        while ((s = __hip_atomic_load(&L->cmd_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == seen) {
+        while ((s = __hip_atomic_load(&L->cmd_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == seen)
             __builtin_amdgcn_s_sleep(2);
-            if (++n >= period) {
-                n = 0;
-                if (aborted()) return;
-            }
-        }
         seen = s;
         Cmd c = L->cmd;
         if (c.op == CMD_EXIT) return;

@@ -94,6 +94,12 @@ __device__ inline int MPI_Initialized(int* flag) {
 }
 __device__ inline int MPI_Finalize(void) {
     using namespace gpumpi::rt;
+    // Requests whose handles were freed still have to complete.
+    Spinner sp;
+    while (ctx()->detached > 0) {
+        progress();
+        if (ctx()->detached == 0 || !sp.keep_going()) break;
+    }
     const bool ok = comm_barrier(world()->comm_world, ctx()->nranks);
     ctx()->finalized = 1;
     return ok ? MPI_SUCCESS : MPI_ERR_OTHER;
@@ -375,15 +381,23 @@ __device__ inline int MPI_Waitsome(int incount, MPI_Request reqv[], int* outcoun
         if (!sp.keep_going()) return MPI_ERR_OTHER;
     }
 }
+// Local and non-blocking: the handle goes away at once, the operation still
+// completes, and progress releases the request when it has (mpi.cu:608-617
+// freed the request object; here the pool entry is detached).
 __device__ inline int MPI_Request_free(MPI_Request* request) {
     using namespace gpumpi::rt;
     if (*request == MPI_REQUEST_NULL) return MPI_SUCCESS;
     const int idx = req_index(*request);
     if (idx < 0) return GPUMPI_FAIL(MPI_ERR_REQUEST, "invalid request handle");
-    // The operation still completes; only the handle goes away.
-    int err = wait_req(idx, MPI_STATUS_IGNORE);
     *request = MPI_REQUEST_NULL;
-    return err;
+    Req* q = &reqs()[idx];
+    if (q->state == RS_COMPLETE) {
+        req_release(idx);
+        return MPI_SUCCESS;
+    }
+    q->flags = (uint8_t)(q->flags | RF_DETACHED);
+    ctx()->detached++;
+    return MPI_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------

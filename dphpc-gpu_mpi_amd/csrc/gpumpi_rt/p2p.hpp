@@ -625,6 +625,15 @@ __device__ inline __attribute__((noinline)) void finish_rendezvous(int idx) {
 // collect rendezvous completions, match receives.  With `copies` (wait and
 // test calls) a second pass then performs the copies of matched rendezvous
 // receives, so every send this rank can post is out before it copies.
+// A request freed by MPI_Request_free is released here once complete.
+__device__ __forceinline__ bool reap_detached(int idx) {
+    Req* q = &reqs()[idx];
+    if (!(q->flags & RF_DETACHED) || q->state != RS_COMPLETE) return false;
+    req_release(idx);
+    ctx()->detached--;
+    return true;
+}
+
 __device__ inline void progress(bool copies) {
     RankLds* L = ctx();
     const int R = L->R;
@@ -633,6 +642,7 @@ __device__ inline void progress(bool copies) {
         int idx;
         if (!ring_live(i, &idx)) continue;
         Req* q = &reqs()[idx];
+        if (L->detached && reap_detached(idx)) continue;
         if (q->kind == REQ_SEND) {
             if (q->state == RS_PENDING) {
                 const uint64_t hb = 1ull << (q->peer & 63);
@@ -662,7 +672,10 @@ __device__ inline void progress(bool copies) {
         int idx;
         if (!ring_live(i, &idx)) continue;
         const Req* q = &reqs()[idx];
-        if (q->kind == REQ_RECV && q->state == RS_MATCHED) finish_rendezvous(idx);
+        if (q->kind == REQ_RECV && q->state == RS_MATCHED) {
+            finish_rendezvous(idx);
+            if (L->detached) reap_detached(idx);
+        }
     }
 }
 

@@ -80,8 +80,8 @@ struct WorldAlloc {
         f(slots); f(chan_head); f(done_words); f(staging); f(arena); f(arena_top);
         f(comm_world); f(comm_self); f(comm_pool); f(comm_pool_top); f(next_ctx); f(rank_ret);
         f(rank_stats); f(dworld); f(world_tables); f(saves); f(members_hbm); f(dev_abort);
-        if (abort_word) (void)hipHostFree(abort_word);
-        if (err) (void)hipHostFree(err);
+        // abort_word and err are process-lifetime host words (host_words()):
+        // never freed here.
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -91,10 +91,31 @@ struct WorldAlloc {
 
 WorldAlloc g_world;
 
+// The abort word and the error record: pinned, coherent host memory, which
+// the GPU reads uncached.  They are allocated once per process and never
+// freed, because device memory allocated after a free of such memory that a
+// kernel has read can later read wrong values on this platform, and every
+// world (re)allocation would otherwise free and re-create them right before
+// the launch that reads the new world.
+struct HostWords {
+    int* abort_word = nullptr;   // [0] host abort request, [1] undelivered-message flag
+    ErrorRecord* err = nullptr;
+};
+HostWords g_host_words;
+
 bool same_shape(const Config& a, const Config& b) {
     return a.nranks == b.nranks && a.chan_slots == b.chan_slots && a.slot_bytes == b.slot_bytes &&
            a.max_requests == b.max_requests && a.staging_bytes == b.staging_bytes &&
            a.arena_bytes == b.arena_bytes && a.comm_pool_bytes == b.comm_pool_bytes;
+}
+
+int host_words(int** abort_word, ErrorRecord** err) {
+    HostWords& H = g_host_words;
+    if (!H.abort_word) HIPOK(hipHostMalloc((void**)&H.abort_word, 64, hipHostMallocCoherent));
+    if (!H.err) HIPOK(hipHostMalloc((void**)&H.err, sizeof(ErrorRecord), hipHostMallocCoherent));
+    *abort_word = H.abort_word;
+    *err = H.err;
+    return GPUMPI_OK;
 }
 
 int alloc_world(const Config& cfg) {
@@ -129,8 +150,10 @@ int alloc_world(const Config& cfg) {
     HIPOK(hipMalloc(&W.members_hbm, P * 2 * P * sizeof(uint64_t)));
     HIPOK(hipMalloc(&W.dev_abort, 256));
     W.need_reset = true;
-    HIPOK(hipHostMalloc((void**)&W.abort_word, 64, hipHostMallocCoherent));
-    HIPOK(hipHostMalloc((void**)&W.err, sizeof(ErrorRecord), hipHostMallocCoherent));
+    {
+        const int rc = host_words(&W.abort_word, &W.err);
+        if (rc) return rc;
+    }
     HIPOK(hipStreamCreateWithFlags(&W.stream, hipStreamNonBlocking));
     HIPOK(hipEventCreate(&W.ev0));
     HIPOK(hipEventCreate(&W.ev1));
@@ -448,6 +471,15 @@ struct Prepared {
 
 int prepare_launch(const gpumpi_config* cfg_in, const void* kernel, int argc, char** argv, void* user_args,
                    gpumpi_result& R, Prepared& L) {
+    // A GPU error left by earlier work (another launch, a copy, a fill) is
+    // reported as such, before anything of this launch runs.
+    {
+        const hipError_t pend = hipDeviceSynchronize();
+        if (pend != hipSuccess) {
+            set_err("GPU error pending before this launch: %s", hipGetErrorString(pend));
+            return GPUMPI_EHIP;
+        }
+    }
     int rc = build_config(cfg_in, &L.cfg, &L.lds);
     if (rc) return rc;
     R.lds_bytes = L.lds;

@@ -104,11 +104,15 @@ GPUMPI_PROGRAM(coll_alltoallv) {
     return 0;
 }
 
-// Gather(v)/Scatter(v)/Allgather(v)/Alltoall over int32 blocks.
+// Gather(v)/Scatter(v)/Allgather(v)/Alltoall(v) over int32 blocks.
 // args: [1] kind (0 gather, 1 gatherv, 2 scatter, 3 scatterv, 4 allgather,
-// 5 allgatherv, 6 alltoall), [2] block elements (int), [3] root,
-// [4] send base, [5] recv base, [6] send stride, [7] recv stride
-// For the v-variants, rank r contributes (block + r) elements.
+// 5 allgatherv, 6 alltoall, 7 alltoall in place, 8 alltoallv in place,
+// 9 alltoall whose send blocks exceed the receive blocks), [2] block
+// elements (int), [3] root, [4] send base, [5] recv base, [6] send stride,
+// [7] recv stride.
+// For the v-variants, rank r contributes (block + r) elements; the in-place
+// Alltoallv exchanges block + (r + k) % 3 elements between ranks r and k
+// (a symmetric count matrix, as MPI requires when sending in place).
 GPUMPI_PROGRAM(coll_blocks) {
     MPI_Init(&argc, &argv);
     int rank, P;
@@ -120,7 +124,7 @@ GPUMPI_PROGRAM(coll_blocks) {
     int* cnt = (int*)gpumpi_malloc(sizeof(int) * 2 * (uint64_t)P);
     int* dsp = cnt + P;
     for (int i = 0; i < P; i++) {
-        cnt[i] = blk + i;
+        cnt[i] = kind == 8 ? blk + (rank + i) % 3 : blk + i;
         dsp[i] = i == 0 ? 0 : dsp[i - 1] + cnt[i - 1];
     }
     switch (kind) {
@@ -130,7 +134,10 @@ GPUMPI_PROGRAM(coll_blocks) {
     case 3: MPI_Scatterv(sb, cnt, dsp, MPI_INT, rb, blk + rank, MPI_INT, root, MPI_COMM_WORLD); break;
     case 4: MPI_Allgather(sb, blk, MPI_INT, rb, blk, MPI_INT, MPI_COMM_WORLD); break;
     case 5: MPI_Allgatherv(sb, blk + rank, MPI_INT, rb, cnt, dsp, MPI_INT, MPI_COMM_WORLD); break;
-    default: MPI_Alltoall(sb, blk, MPI_INT, rb, blk, MPI_INT, MPI_COMM_WORLD); break;
+    case 6: MPI_Alltoall(sb, blk, MPI_INT, rb, blk, MPI_INT, MPI_COMM_WORLD); break;
+    case 7: MPI_Alltoall(MPI_IN_PLACE, 0, MPI_INT, rb, blk, MPI_INT, MPI_COMM_WORLD); break;
+    case 8: MPI_Alltoallv(MPI_IN_PLACE, nullptr, nullptr, MPI_INT, rb, cnt, dsp, MPI_INT, MPI_COMM_WORLD); break;
+    default: MPI_Alltoall(sb, blk + 1, MPI_INT, rb, blk, MPI_INT, MPI_COMM_WORLD); break;
     }
     MPI_Finalize();
     return 0;

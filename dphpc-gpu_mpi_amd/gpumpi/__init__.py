@@ -9,8 +9,10 @@ compiled into lib/libgpumpi_progs.so and looked up by name.
 Importing this module never falls back to anything: if the shared
 libraries are missing, `load()` raises.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -160,6 +162,27 @@ def default_config(nranks, **kw):
     return c
 
 
+# Live device buffers.  At interpreter exit they are freed, and the world
+# released, from an atexit handler -- while the HIP runtime is still up --
+# rather than from __del__ during module teardown, which can run after the
+# runtime's own teardown has begun.
+_live = weakref.WeakSet()
+_shutting_down = False
+
+
+def _release_all():
+    global _shutting_down
+    if _core is None:
+        return
+    for b in list(_live):
+        b.free()
+    _core.gpumpi_release_world()
+    _shutting_down = True
+
+
+atexit.register(_release_all)
+
+
 class DeviceBuffer:
     """A device allocation made through the C-ABI."""
 
@@ -170,6 +193,7 @@ class DeviceBuffer:
         if core.gpumpi_device_alloc(ctypes.byref(p), max(self.nbytes, 1)) != 0:
             raise GpumpiError(last_error())
         self.ptr = p.value
+        _live.add(self)
 
     @classmethod
     def from_array(cls, a):
@@ -197,9 +221,9 @@ class DeviceBuffer:
             raise GpumpiError(last_error())
 
     def free(self):
-        if self.ptr:
+        if self.ptr and not _shutting_down:
             load_core().gpumpi_device_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -243,11 +267,12 @@ def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, **cfg_kw
         res.replay_ms = ms.tolist()
     else:
         rc = core.gpumpi_run(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, ctypes.byref(res))
+    err = last_error() if rc != 0 else ""  # before any later call can overwrite it
     dargs.free()
     if trace:
-        print(f"[gpumpi] done {program} rc={rc} status={res.status}", flush=True)
+        print(f"[gpumpi] done {program} rc={rc} status={res.status} {err}", flush=True)
     if check and rc != 0:
-        raise GpumpiError(f"{program}: {res.describe()} :: {last_error()}")
+        raise GpumpiError(f"{program}: {res.describe()} :: {err}")
     return res
 
 

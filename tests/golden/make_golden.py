@@ -8,6 +8,10 @@ the pi fixture, the reference's cpi.c under /root/reference):
   * cpi_mpich.json   -- the reference's toolchain_tests/pi/cpi.c compiled with
     host mpicc (oracle/Makefile `ref`) and run with N=1e8 at several rank
     counts; the printed pi is recorded.
+  * semantics_mpich.json -- the point-to-point / request / communicator
+    scenarios (dphpc-gpu_mpi_amd/csrc/progs/semantics_scenarios.h) under
+    host MPI (oracle/hostmpi/semantics_golden.c) at 4, 8 and 13 ranks: every
+    rank's record.
 """
 import json
 import os
@@ -26,7 +30,27 @@ def run(cmd, **kw):
     return subprocess.run(cmd, check=True, **kw)
 
 
+def semantics():
+    run(["make", "-C", ORACLE, "_build/hostmpi_semantics_golden"])
+    exe = os.path.join(ORACLE, "_build", "hostmpi_semantics_golden")
+    out = {}
+    for np_ in (4, 8, 13):
+        txt = run([MPIRUN, "-np", str(np_), exe], capture_output=True, text=True).stdout
+        out[str(np_)] = json.loads(txt)
+        # Each scenario alone, so a GPU failure names its scenario.
+        out[str(np_)]["alone"] = []
+        for k in range(8):
+            txt = run([MPIRUN, "-np", str(np_), exe, str(1 << k)], capture_output=True, text=True).stdout
+            out[str(np_)]["alone"].append(json.loads(txt)["records"])
+    with open(os.path.join(HERE, "semantics_mpich.json"), "w") as f:
+        json.dump({"source": "semantics_scenarios.h under MPICH 3.3.2", "runs": out}, f)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "semantics":
+        semantics()
+        return 0
+    semantics()
     run(["make", "-C", ORACLE, "_build/hostmpi_reduce_golden", "ref"])
     run([MPIRUN, "-np", "5", os.path.join(ORACLE, "_build", "hostmpi_reduce_golden"),
          os.path.join(HERE, "reduce_mpich.bin")])

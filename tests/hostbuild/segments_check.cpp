@@ -107,7 +107,9 @@ int main(int argc, char** argv) {
             uint8_t* d;
             const uint8_t* sp;
             uint64_t n;
-            seg_info(c, k, &d, &sp, &n);
+            bool trunc = false;
+            seg_info(c, k, &d, &sp, &n, &trunc);
+            if (trunc) std::printf("unexpected truncation at segment %d of rank %d\n", k, me);
             for (int lane = 0; lane < 64; lane++) {
                 gpumpi_host_lane = lane;
                 copy_part_lanes(d, sp, n, lane, 64);
