@@ -19,10 +19,12 @@
 
 #include "gpumpi/host.h"
 #include "gpumpi/mpi.h"
+#include "host/launcher_internal.hpp"
 
 using namespace gpumpi::rt;
 
-namespace {
+namespace gpumpi {
+namespace host {
 
 thread_local char g_err[512];
 
@@ -32,6 +34,14 @@ void set_err(const char* fmt, ...) {
     vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
 }
+
+}  // namespace host
+}  // namespace gpumpi
+
+using gpumpi::host::g_err;
+using gpumpi::host::set_err;
+
+namespace {
 
 #define HIPOK(call)                                                                            \
     do {                                                                                       \
@@ -242,6 +252,11 @@ int init_world(const Config& cfg, void* user_args) {
     return GPUMPI_OK;
 }
 
+}  // namespace
+
+namespace gpumpi {
+namespace host {
+
 int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
     const gpumpi_config& c = *in;
     if (c.num_procs < 1) { set_err("num_procs must be >= 1"); return GPUMPI_EINVAL; }
@@ -305,7 +320,10 @@ int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
     return GPUMPI_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace gpumpi
+
+using gpumpi::host::build_config;
 
 extern "C" {
 

@@ -35,7 +35,8 @@ OPS = ["MPI_MAX", "MPI_MIN", "MPI_SUM", "MPI_PROD", "MPI_LAND", "MPI_BAND", "MPI
 OP = {n: i for i, n in enumerate(OPS)}
 
 # Exported symbols of each library (the C-ABI surface).
-CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpumpi_run_graph", "gpumpi_main",
+CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpumpi_run_graph", "gpumpi_run_noalloc",
+                "gpumpi_main",
                 "gpumpi_device_alloc", "gpumpi_device_free", "gpumpi_memcpy_htod",
                 "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error",
                 "gpumpi_reset_world", "gpumpi_release_world"]
@@ -98,6 +99,8 @@ def load_core():
                                    vp, ctypes.POINTER(Result)]
         lib.gpumpi_run_graph.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                          vp, ctypes.c_int, ctypes.POINTER(Result), vp]
+        lib.gpumpi_run_noalloc.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_char_p), vp, ctypes.POINTER(Result)]
         lib.gpumpi_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), vp]
         lib.gpumpi_device_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), sz]
         lib.gpumpi_device_free.argtypes = [vp]
@@ -232,13 +235,15 @@ class DeviceBuffer:
             pass
 
 
-def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, **cfg_kw):
+def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, noalloc=False, **cfg_kw):
     """Launch rank program `program` on `nranks` ranks.
 
     `args` are 64-bit words handed to the program (ints or DeviceBuffers,
     which pass their device address).  With graph_replays > 0 the launch is
     captured in a hipGraph and replayed that many times (Result.kernel_ms is
-    the mean replay time; Result.replay_ms the list).  Returns the Result.
+    the mean replay time; Result.replay_ms the list).  With noalloc the
+    launch goes through gpumpi_run_noalloc (the world in the library's static
+    device array).  Returns the Result.
     """
     core, progs = load()
     kern = progs.gpumpi_prog_lookup(program.encode())
@@ -265,6 +270,8 @@ def run(program, nranks, args=(), check=True, argv=(), graph_replays=0, **cfg_kw
         rc = core.gpumpi_run_graph(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, graph_replays,
                                    ctypes.byref(res), ms.ctypes.data)
         res.replay_ms = ms.tolist()
+    elif noalloc:
+        rc = core.gpumpi_run_noalloc(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, ctypes.byref(res))
     else:
         rc = core.gpumpi_run(ctypes.byref(cfg), kern, len(argv), cargv, dargs.ptr, ctypes.byref(res))
     err = last_error() if rc != 0 else ""  # before any later call can overwrite it

@@ -93,6 +93,19 @@ int gpumpi_run(const gpumpi_config* cfg, const void* kernel, int argc, char** ar
 int gpumpi_run_graph(const gpumpi_config* cfg, const void* kernel, int argc, char** argv, void* user_args,
                      int replays, gpumpi_result* result, double* replay_ms);
 
+/* gpumpi_run without any allocation -- no counterpart in the reference; for
+ * callers that cannot allocate (tight serving loops, code that must not call
+ * hipMalloc).  The whole world (channels, staging, rank heaps, communicator
+ * tables, error record, argv) lives in one static device array of 64 MiB per
+ * GPU, so configurations whose world does not fit are refused
+ * (GPUMPI_EINVAL, with the bytes needed in gpumpi_last_error()); use small
+ * staging_bytes / heap_bytes / comm_pool_bytes.  Every call starts from fresh
+ * channel state; it runs on the null stream with no watchdog (timeout_s is
+ * ignored; kernel_ms is host wall time).  Calls that overlap on one GPU
+ * share the array and race. */
+int gpumpi_run_noalloc(const gpumpi_config* cfg, const void* kernel, int argc, char** argv, void* user_args,
+                       gpumpi_result* result);
+
 /* Drop-in host main: parse `---gpumpi`, run, return 0 or 1 like main.cu. */
 int gpumpi_main(int argc, char** argv, const void* kernel);
 
