@@ -49,10 +49,14 @@ __device__ inline void wg_run(const Cmd& c) {
 }
 
 // A device functor run by every thread of the rank (CMD_USER): the functor
-// travels by value inside the LDS command (at most 72 bytes).
+// travels by value inside the LDS command (at most 72 bytes).  It is copied
+// into a local before the call: its captures then live in registers, where
+// the body's own stores cannot alias them (through the storage pointer
+// every store may, and each loop iteration would reload and wait for them).
 template <typename F>
 __device__ void user_trampoline(const void* storage, int t, int nt) {
-    (*(const F*)storage)(t, nt);
+    const F f = *(const F*)storage;
+    f(t, nt);
 }
 template <typename F>
 __device__ inline void par_all_waves(const F& f) {

@@ -117,6 +117,107 @@ __device__ inline void wave_exclusive_scan_to(const int* in, int* out, int n, in
     }
 }
 
+__device__ inline int wave_incl_scan(int v, int lane) {
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+__device__ inline int wave_sum_int(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+// f(key) for every key of keys[0..n): thread t takes the blocks of 16
+// consecutive keys b = t, t + nt, ... (four 16-byte loads in flight per
+// thread, a wave reads 4 KiB contiguous), then the tail one by one.  The
+// same (t, nt) always visits the same keys.  keys is 16-byte aligned.
+template <typename F>
+__device__ __forceinline__ void for_each_key16(const int* keys, int n, int t, int nt, const F& f) {
+    using gpumpi::rt::u32x4;
+    const int nblk = n >> 4;
+    for (int b = t; b < nblk; b += nt) {
+        const u32x4* p = (const u32x4*)(keys + 16 * b);
+        const u32x4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+        f((int)v0.x); f((int)v0.y); f((int)v0.z); f((int)v0.w);
+        f((int)v1.x); f((int)v1.y); f((int)v1.z); f((int)v1.w);
+        f((int)v2.x); f((int)v2.y); f((int)v2.z); f((int)v2.w);
+        f((int)v3.x); f((int)v3.y); f((int)v3.z); f((int)v3.w);
+    }
+    for (int i = (nblk << 4) + t; i < n; i += nt) f(keys[i]);
+}
+
+// Inclusive prefix sum dst[i] = src[0] + ... + src[i] for i < n by every
+// wave of the rank, in two gpumpi_parallel passes.  Wave w owns the
+// contiguous chunk [w*C, (w+1)*C) (C a multiple of 1024); per step a lane
+// holds 16 consecutive elements (four 16-byte loads in flight, the wave
+// covers 4 KiB).  Pass 1 leaves the chunk sums in wsum[]; pass 2 scans each
+// lane's 16 in registers, the lane totals across the wave, and adds the
+// chunk's carry (the sum of the lower chunks).  src may be LDS and dst
+// global memory; src == dst is allowed; both 16-byte aligned.
+__device__ inline int scan_chunk(int n, int nw) { return ((n + nw - 1) / nw + 1023) & ~1023; }
+__device__ inline void load16(const int* src, int i, int hi, int* v) {
+    using gpumpi::rt::u32x4;
+    if (i + 16 <= hi) {
+        const u32x4* p = (const u32x4*)(src + i);
+        const u32x4 a = p[0], b = p[1], c = p[2], d = p[3];
+        v[0] = (int)a.x; v[1] = (int)a.y; v[2] = (int)a.z; v[3] = (int)a.w;
+        v[4] = (int)b.x; v[5] = (int)b.y; v[6] = (int)b.z; v[7] = (int)b.w;
+        v[8] = (int)c.x; v[9] = (int)c.y; v[10] = (int)c.z; v[11] = (int)c.w;
+        v[12] = (int)d.x; v[13] = (int)d.y; v[14] = (int)d.z; v[15] = (int)d.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = i + k < hi ? src[i + k] : 0;
+    }
+}
+__device__ inline void rank_inclusive_scan(const int* src, int* dst, int n, int* wsum) {
+    gpumpi_parallel([=] __device__(int t, int nt) {
+        const int w = t >> 6, lane = t & 63;
+        const int C = scan_chunk(n, nt >> 6);
+        const int lo = w * C, hi = min(n, lo + C);
+        int s = 0;
+        for (int b = lo; b < hi; b += 1024) {
+            int v[16];
+            load16(src, b + 16 * lane, hi, v);
+#pragma unroll
+            for (int k = 0; k < 16; k++) s += v[k];
+        }
+        s = wave_sum_int(s);
+        if (lane == 0) wsum[w] = s;
+    });
+    gpumpi_parallel([=] __device__(int t, int nt) {
+        using gpumpi::rt::u32x4;
+        const int w = t >> 6, lane = t & 63;
+        const int C = scan_chunk(n, nt >> 6);
+        const int lo = w * C, hi = min(n, lo + C);
+        int carry = wave_sum_int(lane < w ? wsum[lane] : 0);
+        for (int b = lo; b < hi; b += 1024) {
+            const int i0 = b + 16 * lane;
+            int v[16];
+            load16(src, i0, hi, v);
+#pragma unroll
+            for (int k = 1; k < 16; k++) v[k] += v[k - 1];
+            const int incl = wave_incl_scan(v[15], lane);
+            const int add = carry + incl - v[15];
+            if (i0 + 16 <= hi) {
+                u32x4* q = (u32x4*)(dst + i0);
+                q[0] = u32x4{(uint32_t)(v[0] + add), (uint32_t)(v[1] + add), (uint32_t)(v[2] + add), (uint32_t)(v[3] + add)};
+                q[1] = u32x4{(uint32_t)(v[4] + add), (uint32_t)(v[5] + add), (uint32_t)(v[6] + add), (uint32_t)(v[7] + add)};
+                q[2] = u32x4{(uint32_t)(v[8] + add), (uint32_t)(v[9] + add), (uint32_t)(v[10] + add),
+                             (uint32_t)(v[11] + add)};
+                q[3] = u32x4{(uint32_t)(v[12] + add), (uint32_t)(v[13] + add), (uint32_t)(v[14] + add),
+                             (uint32_t)(v[15] + add)};
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    if (i0 + k < hi) dst[i0 + k] = v[k] + add;
+            }
+            carry += __shfl(incl, 63);
+        }
+    });
+}
+
 __device__ inline long wave_sum_long(long v) {
     for (int m = 32; m >= 1; m >>= 1) {
         const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)(uint64_t)v, m);
@@ -128,9 +229,15 @@ __device__ inline long wave_sum_long(long v) {
 
 }  // namespace
 
+// Phases of one iteration timed by rank 0 (wall clock, summed over the
+// timed iterations) when args[6] is given.
+enum IsPhase { PH_HIST, PH_PTRS, PH_SCATTER, PH_ALLREDUCE, PH_PLAN, PH_ALLTOALL, PH_ALLTOALLV, PH_SUMS, PH_COUNT,
+               PH_PREFIX, PH_VERIFY, PH_NUM };
+
 // args: [1] class char, [2] out (int64[8]: passed, total keys, errors,
 // reserved...), [3] times out (double[4]: total max, Alltoallv max, ...),
-// [4] sorted keys out (int32[TOTAL_KEYS]) or 0, [5] iterations (default 10)
+// [4] sorted keys out (int32[TOTAL_KEYS]) or 0, [5] iterations (default 10),
+// [6] phase times out (double[PH_NUM], rank 0) or 0
 GPUMPI_PROGRAM(npb_is) {
     MPI_Init(&argc, &argv);
     int my_rank, np_total;
@@ -141,6 +248,17 @@ GPUMPI_PROGRAM(npb_is) {
     double* tout = prog_ptr<double>(3);
     int32_t* keys_out = prog_ptr<int32_t>(4);
     const int max_iter = prog_int(5) > 0 ? (int)prog_int(5) : 10;
+    double* phase_out = prog_ptr<double>(6);
+    uint64_t ph_ticks[PH_NUM] = {};
+    uint64_t ph_t = 0;
+    bool ph_on = false;
+    // Closes the running phase (rank 0, timed iterations only).
+    auto phase = [&](int k) {
+        if (!ph_on) return;
+        const uint64_t now = gpumpi_ticks();
+        ph_ticks[k] += now - ph_t;
+        ph_t = now;
+    };
     IsClass K;
     if (!is_class(cls, &K)) return 1;
     MPI_Comm comm_work;
@@ -167,6 +285,11 @@ GPUMPI_PROGRAM(npb_is) {
     int* recv_displ = send_displ + comm_size;
     int* scratch = (int*)gpumpi_malloc(sizeof(int) * 2048);  // per-thread partials
     int* hist = gpumpi_user_lds<int>();  // NB bucket counters, then (if room) the ranking counts
+    // LDS ints after the bucket counters: the ranking counts of a key range
+    // that fits, and, earlier in each iteration, the per-wave sub-histograms.
+    const int lcap = (int)((gpumpi::rt::ctx()->lds_bytes - gpumpi::rt::ctx()->off_user) / sizeof(int)) - NB;
+    int* sub = hist + NB;
+    const bool sub_hist = lcap >= gpumpi::rt::ctx()->waves * NB;
 
     // create_seq (is.c:535-553) for my slice, in parallel: thread t jumps
     // the generator to its first key.
@@ -199,42 +322,78 @@ GPUMPI_PROGRAM(npb_is) {
             MPI_Barrier(comm_work);
             t_total0 = MPI_Wtime();
             t_a2a = 0;
+            ph_on = phase_out != nullptr && my_rank == 0;
         }
+        if (ph_on) ph_t = gpumpi_ticks();
         if (my_rank == 0) {  // is.c:648-652
             key_array[iteration] = iteration;
             key_array[iteration + 10] = max_key - iteration;
         }
         // bucket counts (is.c:656-674) in LDS, then bucket pointers and the
-        // scatter into buckets (is.c:678-690)
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            for (int i = t; i < NB; i += nt) hist[i] = 0;
-        });
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            for (int i = t; i < num_keys; i += nt) atomicAdd(&hist[key_array[i] >> shift], 1);
-        });
+        // scatter into buckets (is.c:678-690).  With room in the user LDS
+        // every wave counts into its own sub-histogram (atomics contend only
+        // within the wave) and scatters from its own offsets: thread t takes
+        // the same keys in both passes, so wave w's offsets in bucket b are
+        // bucket_ptrs[b] plus the lower waves' counts of b.
+        if (sub_hist) {
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < (nt >> 6) * NB; i += nt) sub[i] = 0;
+            });
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                int* h = sub + (t >> 6) * NB;
+                for_each_key16(key_array, num_keys, t, nt, [=](int k) { atomicAdd(&h[k >> shift], 1); });
+            });
+        } else {
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for (int i = t; i < NB; i += nt) hist[i] = 0;
+            });
+            gpumpi_parallel([=] __device__(int t, int nt) {
+                for_each_key16(key_array, num_keys, t, nt, [=](int k) { atomicAdd(&hist[k >> shift], 1); });
+            });
+        }
+        phase(PH_HIST);
         for (int i = 0; i < 5; i++) bucket_size[NB + i] = 0;
         for (int i = 0; i < 5; i++)
             if (K.test_index[i] / num_keys == my_rank) bucket_size[NB + i] = key_array[K.test_index[i] % num_keys];
         gpumpi_parallel([=] __device__(int t, int nt) {
+            const int nw = nt >> 6;
             for (int i = t; i < NB; i += nt) {
-                bucket_size[i] = hist[i];
-                bucket_ptrs[i] = hist[i];
+                int c = 0;
+                if (sub_hist)
+                    for (int w = 0; w < nw; w++) c += sub[w * NB + i];
+                else
+                    c = hist[i];
+                bucket_size[i] = c;
+                bucket_ptrs[i] = c;
             }
         });
         gpumpi_parallel([=] __device__(int t, int nt) {
             if (t < 64) wave_exclusive_scan(bucket_ptrs, NB, t);
         });
         gpumpi_parallel([=] __device__(int t, int nt) {
-            for (int i = t; i < NB; i += nt) hist[i] = bucket_ptrs[i];
-        });
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            for (int i = t; i < num_keys; i += nt) {
-                const int key = key_array[i];
-                key_buff1[atomicAdd(&hist[key >> shift], 1)] = key;
+            const int nw = nt >> 6;
+            for (int i = t; i < NB; i += nt) {
+                if (sub_hist) {
+                    int run = bucket_ptrs[i];
+                    for (int w = 0; w < nw; w++) {
+                        const int c = sub[w * NB + i];
+                        sub[w * NB + i] = run;
+                        run += c;
+                    }
+                } else {
+                    hist[i] = bucket_ptrs[i];
+                }
             }
         });
+        phase(PH_PTRS);
+        gpumpi_parallel([=] __device__(int t, int nt) {
+            int* h = sub_hist ? sub + (t >> 6) * NB : hist;
+            for_each_key16(key_array, num_keys, t, nt, [=](int k) { key_buff1[atomicAdd(&h[k >> shift], 1)] = k; });
+        });
+        phase(PH_SCATTER);
         // is.c:696-701
         MPI_Allreduce(bucket_size, bucket_totals, NB + 5, MPI_INT, MPI_SUM, comm_work);
+        phase(PH_ALLREDUCE);
         // redistribution plan, is.c:706-751 (sequential, as in the benchmark)
         {
             long bucket_sum_accumulator = 0;
@@ -297,6 +456,7 @@ GPUMPI_PROGRAM(npb_is) {
                 j++;
             }
         }
+        phase(PH_PLAN);
         // is.c:758-781
         const double a0 = MPI_Wtime();
         MPI_Alltoall(send_count, 1, MPI_INT, recv_count, 1, MPI_INT, comm_work);
@@ -304,9 +464,11 @@ GPUMPI_PROGRAM(npb_is) {
         // program's wave in one pass instead of a loop of dependent loads.
         wave_exclusive_scan_to(recv_count, recv_displ, comm_size, gpumpi::rt::lane_id());
         gpumpi::rt::wave_sync();  // MPI_Alltoallv's lanes read entries other lanes wrote
+        phase(PH_ALLTOALL);
         MPI_Alltoallv(key_buff1, send_count, send_displ, MPI_INT, key_buff2, recv_count, recv_displ, MPI_INT,
                       comm_work);
         t_a2a += MPI_Wtime() - a0;
+        phase(PH_ALLTOALLV);
         // local ranking, is.c:787-834
         const int min_key_val = ptr1[my_rank] << shift;
         const int max_key_val = ((ptr2[my_rank] + 1) << shift) - 1;
@@ -320,50 +482,36 @@ GPUMPI_PROGRAM(npb_is) {
         long jl = 0;
         for (int i = ptr1[my_rank] + gpumpi::rt::lane_id(); i <= ptr2[my_rank]; i += 64) jl += bucket_totals[i];
         const int nloc = (int)wave_sum_long(jl);
+        phase(PH_SUMS);
         // Key counts over my key range: in LDS when the range fits beside the
         // bucket counters (the user LDS the launch reserved), then copied to
         // `counts`; else straight in global memory.
         int* lcounts = hist + NB;
-        const int lcap = (int)((gpumpi::rt::ctx()->lds_bytes - gpumpi::rt::ctx()->off_user) / sizeof(int)) - NB;
-        if (range <= lcap) {
+        const bool counts_in_lds = range <= lcap;
+        if (counts_in_lds) {
             gpumpi_parallel([=] __device__(int t, int nt) {
                 for (int i = t; i < range; i += nt) lcounts[i] = 0;
             });
             gpumpi_parallel([=] __device__(int t, int nt) {
-                for (int i = t; i < nloc; i += nt) atomicAdd(&lcounts[key_buff2[i] - min_key_val], 1);
-            });
-            gpumpi_parallel([=] __device__(int t, int nt) {
-                for (int i = t; i < range; i += nt) counts[i] = lcounts[i];
+                for_each_key16(key_buff2, nloc, t, nt, [=](int k) { atomicAdd(&lcounts[k - min_key_val], 1); });
             });
         } else {
+            // 16-byte zero stores (counts is 256-byte aligned), then the tail
             gpumpi_parallel([=] __device__(int t, int nt) {
-                for (int i = t; i < range; i += nt) counts[i] = 0;
+                const int n4 = range >> 2;
+                const gpumpi::rt::u32x4 z = {0u, 0u, 0u, 0u};
+                for (int i = t; i < n4; i += nt) ((gpumpi::rt::u32x4*)counts)[i] = z;
+                for (int i = (n4 << 2) + t; i < range; i += nt) counts[i] = 0;
             });
             gpumpi_parallel([=] __device__(int t, int nt) {
-                for (int i = t; i < nloc; i += nt) atomicAdd(&counts[key_buff2[i] - min_key_val], 1);
+                for_each_key16(key_buff2, nloc, t, nt, [=](int k) { atomicAdd(&counts[k - min_key_val], 1); });
             });
         }
-        // inclusive prefix over the key range (is.c:829-834): per-thread
-        // chunk sums, a wave scan of the sums, then the chunk scans.
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            const int chunk = (range + nt - 1) / nt;
-            const int lo = t * chunk, hi = min(range, lo + chunk);
-            int s = 0;
-            for (int i = lo; i < hi; i++) s += counts[i];
-            scratch[t] = s;
-        });
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            if (t < 64) wave_exclusive_scan(scratch, nt, t);
-        });
-        gpumpi_parallel([=] __device__(int t, int nt) {
-            const int chunk = (range + nt - 1) / nt;
-            const int lo = t * chunk, hi = min(range, lo + chunk);
-            int run = scratch[t];
-            for (int i = lo; i < hi; i++) {
-                run += counts[i];
-                counts[i] = run;
-            }
-        });
+        phase(PH_COUNT);
+        // inclusive prefix over the key range (is.c:829-834), coalesced per
+        // wave; from the LDS counts straight into `counts` when they are there
+        rank_inclusive_scan(counts_in_lds ? lcounts : counts, counts, range, scratch);
+        phase(PH_PREFIX);
         // partial verification, is.c:840-913
         for (int i = 0; i < 5; i++) {
             const int k = bucket_totals[i + NB];
@@ -377,7 +525,10 @@ GPUMPI_PROGRAM(npb_is) {
             total_local_keys = nloc;
             min_key_last = min_key_val;
         }
+        phase(PH_VERIFY);
     }
+    if (phase_out && my_rank == 0 && gpumpi_is_leader_lane())
+        for (int k = 0; k < PH_NUM; k++) phase_out[k] = (double)ph_ticks[k] * MPI_Wtick();
     const double timecounter = MPI_Wtime() - t_total0;
     double maxtime = 0, maxa2a = 0;
     MPI_Reduce(&timecounter, &maxtime, 1, MPI_DOUBLE, MPI_MAX, 0, comm_work);
@@ -392,14 +543,15 @@ GPUMPI_PROGRAM(npb_is) {
                 key_array[atomicSub(&counts[key - mk], 1) - 1] = key;
             }
         });
-        int k = 0;
+        int* kp = scratch + 2047;  // the lower neighbour's last key
+        *kp = 0;
         MPI_Request request = MPI_REQUEST_NULL;
         const int last_local_key = nloc < 1 ? 0 : nloc - 1;
-        if (my_rank > 0) MPI_Irecv(&k, 1, MPI_INT, my_rank - 1, 1000, comm_work, &request);
+        if (my_rank > 0) MPI_Irecv(kp, 1, MPI_INT, my_rank - 1, 1000, comm_work, &request);
         if (my_rank < comm_size - 1) MPI_Send(&key_array[last_local_key], 1, MPI_INT, my_rank + 1, 1000, comm_work);
         if (my_rank > 0) MPI_Wait(&request, MPI_STATUS_IGNORE);
         int j = 0;
-        if (my_rank > 0 && nloc > 0 && k > key_array[0]) j++;
+        if (my_rank > 0 && nloc > 0 && *kp > key_array[0]) j++;
         gpumpi_parallel([=] __device__(int t, int nt) {
             int bad = 0;
             for (int i = 1 + t; i < nloc; i += nt) bad += key_array[i - 1] > key_array[i];

@@ -69,6 +69,7 @@ typedef struct orc_is_result {
     int full_verify_errors;   /* keys out of order, summed over ranks */
     int64_t total_keys_after; /* sum of total_local_keys over ranks */
     uint64_t sorted_checksum; /* sum over ranks of sum_i mix64(key_i + i*2^32) */
+    double timed_seconds;     /* wall time of the 10 timed iterations (one host thread) */
 } orc_is_result;
 
 /* Runs the whole IS benchmark (one untimed + 10 iterations + full_verify)

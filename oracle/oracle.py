@@ -49,7 +49,8 @@ class ISResult(ctypes.Structure):
     _fields_ = [("passed_verification", ctypes.c_int),
                 ("full_verify_errors", ctypes.c_int),
                 ("total_keys_after", ctypes.c_int64),
-                ("sorted_checksum", ctypes.c_uint64)]
+                ("sorted_checksum", ctypes.c_uint64),
+                ("timed_seconds", ctypes.c_double)]
 
 
 def load():

@@ -85,3 +85,31 @@ def test_sim_abort_leaves_unpublished_tables_alone(sim):
     rc, out = _run([os.path.join(sim, "run_coll"), "abort", "8", "128"])
     assert rc == 0, out[-3000:]
     assert "mpi_error=7" in out and "AddressSanitizer" not in out
+
+
+@pytest.fixture(scope="module")
+def sim_is():
+    out = os.path.join(HERE, "hostbuild", "_build", "simt_is")
+    env = dict(os.environ, SIMT_PROGS="npb_is.hip")
+    subprocess.run(["bash", os.path.join(SIM, "build.sh"), out, os.path.join(SIM, "run_is.cpp")], check=True,
+                   env=env, timeout=1200)
+    return out
+
+
+def _keyhash(keys):
+    h = 1469598103934665603
+    for x in keys.astype(np.uint32).tolist():
+        h = ((h ^ x) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+# (class, ranks, user LDS bytes): 4096 keeps the counts and the shared
+# bucket histogram in global memory / one LDS array; the larger sizes put
+# per-wave sub-histograms and (where the key range fits) the ranking counts
+# in LDS.
+@pytest.mark.parametrize("cls,P,ulds", [("S", 4, 4096), ("S", 4, 20480), ("S", 16, 20480), ("W", 4, 36864)])
+def test_sim_npb_is_matches_oracle(sim_is, orc, cls, P, ulds):
+    res, keys, _ = orc.is_run(cls, P, want_keys=True)
+    rc, out = _run([os.path.join(sim_is, "run_is"), cls, str(P), "128", str(ulds)])
+    assert rc == 0, out[-3000:]
+    assert (f"passed={res.passed_verification} total={keys.size} errors=0 keyhash={_keyhash(keys)}") in out, out[-2000:]
