@@ -196,6 +196,11 @@ IS_PHASES = ["bucket histogram", "bucket pointers", "scatter to buckets", "Allre
              "count prefix", "partial verification"]
 
 
+# The largest user LDS of a 128-rank launch (160 KiB less the runtime's part):
+# the widest LDS windows for the edge ranks' key counts.
+IS_USER_LDS = 151552
+
+
 def is_config(gm, np, threads, cpu):
     """NPB IS class B at 128 ranks: keys/s over the 10 timed iterations
     (is.c:1148), its Alltoallv's GB/s on 2 * 2^25 * 4 bytes per iteration,
@@ -206,7 +211,7 @@ def is_config(gm, np, threads, cpu):
     ph = gm.DeviceBuffer(8 * len(IS_PHASES))
     ph.zero()
     gm.run("npb_is", 128, [0, ord("B"), res, tout, 0, 10, ph], threads_per_rank=threads,
-           user_lds_bytes=4096 + 98304, heap_bytes=128 * (32 << 20), max_requests=64, timeout_s=120.0)
+           user_lds_bytes=IS_USER_LDS, heap_bytes=128 * (32 << 20), max_requests=64, timeout_s=120.0)
     r = res.download(np.int64)
     tt = tout.download(np.float64)
     phases = ph.download(np.float64)

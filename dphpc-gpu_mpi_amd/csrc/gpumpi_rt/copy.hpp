@@ -28,6 +28,11 @@ constexpr int kCopyUnroll = 8;
 #ifndef GPUMPI_GLOBAL
 #define GPUMPI_GLOBAL __attribute__((address_space(1)))
 #endif
+// LDS (local data share) pointers: atomics and loads through them are ds_*
+// instructions rather than flat ones (host builds predefine it empty).
+#ifndef GPUMPI_LDS
+#define GPUMPI_LDS __attribute__((address_space(3)))
+#endif
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16-byte vector, plain value type
 typedef GPUMPI_GLOBAL u32x4 gvec4;
 

@@ -129,15 +129,43 @@ __device__ inline int wave_sum_int(int v) {
     return v;
 }
 
+// Atomic add among the threads of this rank's workgroup only (the ranking
+// counts are the rank's own): workgroup scope lets the cache hierarchy
+// serve it, where an agent-scope atomic is carried out past the L2.
+__device__ __forceinline__ int wg_atomic_add(int* p, int v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// The same on a pointer known to be the rank's LDS (user LDS): ds_add.
+typedef GPUMPI_LDS int lds_int;
+__device__ __forceinline__ int lds_atomic_add(int* p, int v) {
+    return __hip_atomic_fetch_add((lds_int*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // f(key) for every key of keys[0..n): thread t takes the blocks of 16
-// consecutive keys b = t, t + nt, ... (four 16-byte loads in flight per
-// thread, a wave reads 4 KiB contiguous), then the tail one by one.  The
-// same (t, nt) always visits the same keys.  keys is 16-byte aligned.
-template <typename F>
+// consecutive keys b = t, t + nt, ... (a wave reads 4 KiB contiguous per
+// block), D blocks per step so that 4*D 16-byte loads per thread are in
+// flight together, then the tail one by one.  The same (t, nt) visits the
+// same keys in the same order whatever D.  keys is 16-byte aligned.
+template <int D = 1, typename F>
 __device__ __forceinline__ void for_each_key16(const int* keys, int n, int t, int nt, const F& f) {
     using gpumpi::rt::u32x4;
     const int nblk = n >> 4;
-    for (int b = t; b < nblk; b += nt) {
+    int b = t;
+    for (; b + (D - 1) * nt < nblk; b += D * nt) {
+        u32x4 v[D][4];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const u32x4* p = (const u32x4*)(keys + 16 * (b + d * nt));
+            v[d][0] = p[0]; v[d][1] = p[1]; v[d][2] = p[2]; v[d][3] = p[3];
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                f((int)v[d][q].x); f((int)v[d][q].y); f((int)v[d][q].z); f((int)v[d][q].w);
+            }
+    }
+    for (; b < nblk; b += nt) {
         const u32x4* p = (const u32x4*)(keys + 16 * b);
         const u32x4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
         f((int)v0.x); f((int)v0.y); f((int)v0.z); f((int)v0.w);
@@ -154,8 +182,8 @@ __device__ __forceinline__ void for_each_key16(const int* keys, int n, int t, in
 // holds 16 consecutive elements (four 16-byte loads in flight, the wave
 // covers 4 KiB).  Pass 1 leaves the chunk sums in wsum[]; pass 2 scans each
 // lane's 16 in registers, the lane totals across the wave, and adds the
-// chunk's carry (the sum of the lower chunks).  src may be LDS and dst
-// global memory; src == dst is allowed; both 16-byte aligned.
+// chunk's carry (the sum of the lower chunks) and carry0.  src may be LDS
+// and dst global memory; src == dst is allowed; both 16-byte aligned.
 __device__ inline int scan_chunk(int n, int nw) { return ((n + nw - 1) / nw + 1023) & ~1023; }
 __device__ inline void load16(const int* src, int i, int hi, int* v) {
     using gpumpi::rt::u32x4;
@@ -171,7 +199,7 @@ __device__ inline void load16(const int* src, int i, int hi, int* v) {
         for (int k = 0; k < 16; k++) v[k] = i + k < hi ? src[i + k] : 0;
     }
 }
-__device__ inline void rank_inclusive_scan(const int* src, int* dst, int n, int* wsum) {
+__device__ inline void rank_inclusive_scan(const int* src, int* dst, int n, int* wsum, int carry0 = 0) {
     gpumpi_parallel([=] __device__(int t, int nt) {
         const int w = t >> 6, lane = t & 63;
         const int C = scan_chunk(n, nt >> 6);
@@ -191,7 +219,7 @@ __device__ inline void rank_inclusive_scan(const int* src, int* dst, int n, int*
         const int w = t >> 6, lane = t & 63;
         const int C = scan_chunk(n, nt >> 6);
         const int lo = w * C, hi = min(n, lo + C);
-        int carry = wave_sum_int(lane < w ? wsum[lane] : 0);
+        int carry = carry0 + wave_sum_int(lane < w ? wsum[lane] : 0);
         for (int b = lo; b < hi; b += 1024) {
             const int i0 = b + 16 * lane;
             int v[16];
@@ -341,14 +369,14 @@ GPUMPI_PROGRAM(npb_is) {
             });
             gpumpi_parallel([=] __device__(int t, int nt) {
                 int* h = sub + (t >> 6) * NB;
-                for_each_key16(key_array, num_keys, t, nt, [=](int k) { atomicAdd(&h[k >> shift], 1); });
+                for_each_key16<2>(key_array, num_keys, t, nt, [=](int k) { lds_atomic_add(&h[k >> shift], 1); });
             });
         } else {
             gpumpi_parallel([=] __device__(int t, int nt) {
                 for (int i = t; i < NB; i += nt) hist[i] = 0;
             });
             gpumpi_parallel([=] __device__(int t, int nt) {
-                for_each_key16(key_array, num_keys, t, nt, [=](int k) { atomicAdd(&hist[k >> shift], 1); });
+                for_each_key16<2>(key_array, num_keys, t, nt, [=](int k) { lds_atomic_add(&hist[k >> shift], 1); });
             });
         }
         phase(PH_HIST);
@@ -388,7 +416,7 @@ GPUMPI_PROGRAM(npb_is) {
         phase(PH_PTRS);
         gpumpi_parallel([=] __device__(int t, int nt) {
             int* h = sub_hist ? sub + (t >> 6) * NB : hist;
-            for_each_key16(key_array, num_keys, t, nt, [=](int k) { key_buff1[atomicAdd(&h[k >> shift], 1)] = k; });
+            for_each_key16<2>(key_array, num_keys, t, nt, [=](int k) { key_buff1[lds_atomic_add(&h[k >> shift], 1)] = k; });
         });
         phase(PH_SCATTER);
         // is.c:696-701
@@ -484,17 +512,41 @@ GPUMPI_PROGRAM(npb_is) {
         const int nloc = (int)wave_sum_long(jl);
         phase(PH_SUMS);
         // Key counts over my key range: in LDS when the range fits beside the
-        // bucket counters (the user LDS the launch reserved), then copied to
-        // `counts`; else straight in global memory.
+        // bucket counters (the user LDS the launch reserved), then scanned
+        // into `counts`.  A wider range (the edge ranks of the key
+        // distribution) is counted in LDS windows of wn keys, one pass over
+        // the received keys per window, each window scanned straight into its
+        // part of `counts` with the keys below it as carry.  Without room
+        // for such windows: atomics on `counts` in global memory.
         int* lcounts = hist + NB;
+        const int wn = lcap & ~15;
         const bool counts_in_lds = range <= lcap;
+        const bool windowed = !counts_in_lds && wn >= 1024;
         if (counts_in_lds) {
             gpumpi_parallel([=] __device__(int t, int nt) {
                 for (int i = t; i < range; i += nt) lcounts[i] = 0;
             });
             gpumpi_parallel([=] __device__(int t, int nt) {
-                for_each_key16(key_buff2, nloc, t, nt, [=](int k) { atomicAdd(&lcounts[k - min_key_val], 1); });
+                for_each_key16<2>(key_buff2, nloc, t, nt, [=](int k) { lds_atomic_add(&lcounts[k - min_key_val], 1); });
             });
+        } else if (windowed) {
+            int carry = 0;
+            for (int lo = 0; lo < range; lo += wn) {
+                const int n = min(wn, range - lo);
+                const int base = min_key_val + lo;
+                gpumpi_parallel([=] __device__(int t, int nt) {
+                    for (int i = t; i < n; i += nt) lcounts[i] = 0;
+                });
+                gpumpi_parallel([=] __device__(int t, int nt) {
+                    for_each_key16<3>(key_buff2, nloc, t, nt, [=](int k) {
+                        const unsigned d = (unsigned)(k - base);
+                        if (d < (unsigned)n) lds_atomic_add(&lcounts[d], 1);
+                    });
+                });
+                // counts + lo stays 16-byte aligned: wn is a multiple of 16
+                rank_inclusive_scan(lcounts, counts + lo, n, scratch, carry);
+                carry = counts[lo + n - 1];
+            }
         } else {
             // 16-byte zero stores (counts is 256-byte aligned), then the tail
             gpumpi_parallel([=] __device__(int t, int nt) {
@@ -504,13 +556,13 @@ GPUMPI_PROGRAM(npb_is) {
                 for (int i = (n4 << 2) + t; i < range; i += nt) counts[i] = 0;
             });
             gpumpi_parallel([=] __device__(int t, int nt) {
-                for_each_key16(key_buff2, nloc, t, nt, [=](int k) { atomicAdd(&counts[k - min_key_val], 1); });
+                for_each_key16(key_buff2, nloc, t, nt, [=](int k) { wg_atomic_add(&counts[k - min_key_val], 1); });
             });
         }
         phase(PH_COUNT);
         // inclusive prefix over the key range (is.c:829-834), coalesced per
         // wave; from the LDS counts straight into `counts` when they are there
-        rank_inclusive_scan(counts_in_lds ? lcounts : counts, counts, range, scratch);
+        if (!windowed) rank_inclusive_scan(counts_in_lds ? lcounts : counts, counts, range, scratch);
         phase(PH_PREFIX);
         // partial verification, is.c:840-913
         for (int i = 0; i < 5; i++) {
@@ -540,7 +592,7 @@ GPUMPI_PROGRAM(npb_is) {
         gpumpi_parallel([=] __device__(int t, int nt) {
             for (int i = t; i < nloc; i += nt) {
                 const int key = key_buff2[i];
-                key_array[atomicSub(&counts[key - mk], 1) - 1] = key;
+                key_array[wg_atomic_add(&counts[key - mk], -1) - 1] = key;
             }
         });
         int* kp = scratch + 2047;  // the lower neighbour's last key

@@ -23,6 +23,7 @@
 #define __launch_bounds__(...)
 #define __shared__ thread_local
 #define GPUMPI_GLOBAL
+#define GPUMPI_LDS
 #define gpumpi_lds_base (*gpumpi_host_lds)
 
 extern thread_local int gpumpi_host_lane;

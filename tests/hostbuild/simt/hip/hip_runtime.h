@@ -25,6 +25,7 @@
 #define __launch_bounds__(...)
 #define __shared__
 #define GPUMPI_GLOBAL
+#define GPUMPI_LDS
 #define gpumpi_lds_base (*gpumpi_host_lds)
 
 struct dim3 {

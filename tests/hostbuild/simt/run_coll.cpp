@@ -4,11 +4,13 @@
 //   run_coll blocks KIND THREADS               (P = 6, block 5, root 2)
 //   run_coll abort P THREADS                   (rank 1 aborts; the others keep
 //                                               calling collectives)
+//   run_coll wire FMT P N THREADS              (compressed-wire Allreduce)
 #include "progs/registry.hpp"
 
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -144,6 +146,49 @@ static int blocks(int kind, int threads) {
     return bad ? 1 : 0;
 }
 
+// Compressed-wire Allreduce (coll_reduce with args[13] = wire) of P rows of n
+// floats: every rank's output must be the same and within the wire's error
+// of the exact sum.
+static int wire(int wire_fmt, int P, int n, int threads) {
+    const uint64_t stride = ((uint64_t)n * 4 + 255) / 256 * 256;
+    uint8_t *ib = nullptr, *ob = nullptr;
+    if (gpumpi_device_alloc((void**)&ib, stride * P) || gpumpi_device_alloc((void**)&ob, stride * P)) return 4;
+    std::memset(ob, 0, stride * P);
+    std::vector<double> exact(n, 0.0);
+    for (int r = 0; r < P; r++)
+        for (int i = 0; i < n; i++) {
+            const float v = (float)((((int64_t)r * 7919 + (int64_t)i * 104729) % 2001) - 1000) / 37.0f;
+            ((float*)(ib + r * stride))[i] = v;
+            exact[i] += v;
+        }
+    std::vector<uint64_t> words(64, 0);
+    words[1] = 12;  // MPI_FLOAT (datatype enum order)
+    words[2] = 2;   // MPI_SUM (operator index)
+    words[3] = (uint64_t)n;
+    words[4] = (uint64_t)(uintptr_t)ib;
+    words[5] = (uint64_t)(uintptr_t)ob;
+    words[6] = stride;
+    words[7] = (uint64_t)(int64_t)-1;  // no root: Allreduce
+    words[9] = 1;
+    words[10] = 1;
+    words[13] = (uint64_t)wire_fmt;
+    if (launch("coll_reduce", P, threads, words)) return 1;
+    long bad = 0;
+    double e2 = 0, x2 = 0;
+    for (int r = 0; r < P; r++)
+        for (int i = 0; i < n; i++) {
+            const float g = ((float*)(ob + r * stride))[i];
+            bad += std::memcmp(&g, ob + i * 4, 4) != 0;
+            if (r == 0) {
+                e2 += (g - exact[i]) * (g - exact[i]);
+                x2 += exact[i] * exact[i];
+            }
+        }
+    const double rel = x2 > 0 ? std::sqrt(e2 / x2) : 0;
+    std::printf("wire %d P=%d n=%d: rank mismatches %ld, relative error %.3g\n", wire_fmt, P, n, bad, rel);
+    return bad == 0 && rel < (wire_fmt == 1 ? 1e-2 : 6e-2) ? 0 : 1;
+}
+
 static int abort_case(int P, int threads) {
     std::vector<uint64_t> words(64, 0);
     int err = 0;
@@ -156,6 +201,8 @@ int main(int argc, char** argv) {
     if (argc >= 4 && !std::strcmp(argv[1], "abort")) return abort_case(std::atoi(argv[2]), std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "alltoallv")) return alltoallv(argv[2], std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "blocks")) return blocks(std::atoi(argv[2]), std::atoi(argv[3]));
+    if (argc >= 6 && !std::strcmp(argv[1], "wire"))
+        return wire(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     std::fprintf(stderr, "usage: run_coll alltoallv COUNTS THREADS | blocks KIND THREADS | abort P THREADS\n");
     return 2;
 }

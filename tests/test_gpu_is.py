@@ -11,17 +11,20 @@ TOTAL = {"S": 1 << 16, "W": 1 << 20, "A": 1 << 23, "B": 1 << 25}
 
 
 # Classes A and B get room in LDS for the ranking counts (the program's LDS
-# path); S and W keep only the bucket counters (its global-memory path).
+# path for the middle ranks, LDS windows for the edge ranks' wide key
+# ranges); S and W keep only the bucket counters (its global-memory path).
 USER_LDS = {"S": 4096, "W": 4096, "A": 4096 + 98304, "B": 4096 + 98304}
+# The largest user LDS a 128-rank launch allows (the bench's setting).
+USER_LDS_MAX_128 = 151552
 
 
-def run_is(gm, cls, P, want_keys=True, threads=256):
+def run_is(gm, cls, P, want_keys=True, threads=256, user_lds=None):
     out = gm.DeviceBuffer(8 * 8)
     out.zero()
     tout = gm.DeviceBuffer(8 * 4)
     keys = gm.DeviceBuffer(4 * TOTAL[cls]) if want_keys else None
     res = gm.run("npb_is", P, [0, ord(cls), out, tout, keys, 10], threads_per_rank=threads,
-                 user_lds_bytes=USER_LDS[cls], heap_bytes=P * (32 << 20) if cls == "B" else P * (64 << 20) // max(1, P // 8),
+                 user_lds_bytes=user_lds or USER_LDS[cls], heap_bytes=P * (32 << 20) if cls == "B" else P * (64 << 20) // max(1, P // 8),
                  max_requests=64, timeout_s=120.0)
     assert res.status == 0, res.describe()
     o = out.download(np.int64)
@@ -42,9 +45,10 @@ def test_is_matches_oracle(gm, orc, cls, P):
     assert np.array_equal(keys, okeys)
 
 
-def test_is_class_b_128(gm, orc):
-    print("[gpu] npb_is class B P=128", flush=True)
-    o, t, keys = run_is(gm, "B", 128)
+@pytest.mark.parametrize("user_lds", [USER_LDS["B"], USER_LDS_MAX_128])
+def test_is_class_b_128(gm, orc, user_lds):
+    print(f"[gpu] npb_is class B P=128 user_lds={user_lds}", flush=True)
+    o, t, keys = run_is(gm, "B", 128, user_lds=user_lds)
     assert o[0] == 50 + 128, o
     assert o[2] == 0 and o[1] == 1 << 25
     res, okeys, _ = orc.is_run("B", 128, want_keys=True)

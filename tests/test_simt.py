@@ -2,7 +2,7 @@
 (tests/hostbuild/simt): every lane a fiber, lockstep wave operations, built
 under AddressSanitizer and UBSan.  These run the launcher and the programs
 end to end without a GPU: the batched uneven Allreduce (exact, BF16 and FP8
-wires), the uneven Alltoallv and the block collectives at the shapes of the
+wires), the compressed-wire Allreduce, the uneven Alltoallv and the block collectives at the shapes of the
 GPU tests, and a launch in which one rank aborts while the others keep
 calling collectives."""
 import os
@@ -76,6 +76,15 @@ def test_sim_alltoallv_uneven(sim, tmp_path, P):
 def test_sim_block_collectives(sim, kind):
     rc, out = _run([os.path.join(sim, "run_coll"), "blocks", str(kind), "128"])
     assert rc == 0 and f"blocks kind {kind} mismatches: 0" in out, out[-3000:]
+
+
+@pytest.mark.parametrize("wire,P,n", [(1, 4, 1000), (2, 4, 1000), (1, 9, 70001), (2, 9, 70001)])
+def test_sim_wire_allreduce(sim, wire, P, n):
+    # The compressed-wire Allreduce at the GPU test's shapes: every rank's
+    # output identical, within the wire's error of the exact sum, and no
+    # out-of-bounds access under AddressSanitizer.
+    rc, out = _run([os.path.join(sim, "run_coll"), "wire", str(wire), str(P), str(n), "256"])
+    assert rc == 0 and f"wire {wire} P={P} n={n}: rank mismatches 0" in out, out[-3000:]
 
 
 def test_sim_abort_leaves_unpublished_tables_alone(sim):
