@@ -7,7 +7,11 @@
 // fibers meet at __syncthreads, and s_sleep yields.  One OS thread runs all
 // fibers cooperatively, so memory is sequentially consistent: the
 // simulation finds out-of-bounds and wild accesses, wrong indexing and
-// protocol deadlocks, not memory-ordering races.
+// protocol deadlocks.  Memory-ordering races between ranks are found by a
+// happens-before checker (SIMT_RACE=1, simt.cpp): every scoped atomic and
+// fence below reports to it, and every plain load and store of global
+// memory is checked against the accesses of other ranks it is not ordered
+// with by agent-scope release/acquire synchronisation.
 #pragma once
 
 #include <cmath>
@@ -48,6 +52,7 @@ struct Fiber {
     uint64_t nops = 0;  // wave operations this lane has taken part in
     const uint64_t* wait_gen = nullptr;  // blocked until *wait_gen != wait_val
     uint64_t wait_val = 0;
+    bool in_atomic = false;  // inside a scoped atomic: its access is not a plain one
 };
 extern Fiber* cur;
 extern dim3 grid_dim, block_dim;
@@ -62,6 +67,10 @@ bool is_private(const void* p);
 bool is_shared(const void* p);
 uint64_t steady_counter();
 [[noreturn]] void fail(const char* what);
+// Happens-before checker hooks (no-ops unless SIMT_RACE=1).
+enum RaceKind { RACE_LOAD = 0, RACE_STORE = 1, RACE_RMW = 2 };
+void race_fence(int order, const char* scope);
+void race_atomic(const void* p, int kind, int order, int scope);
 }  // namespace simt
 
 #define threadIdx (::simt::cur->tid)
@@ -106,7 +115,7 @@ template <typename T>
 inline T simt_readfirstlane(T v) { return simt_from<T>(::simt::wave_op(::simt::OP_FIRST, simt_bits(v), 0)); }
 #define __builtin_amdgcn_readfirstlane(v) simt_readfirstlane(v)
 #define __builtin_amdgcn_readlane(v, l) __shfl((v), (l))
-#define __builtin_amdgcn_fence(order, scope) __atomic_signal_fence(order)
+#define __builtin_amdgcn_fence(order, scope) ::simt::race_fence((order), (scope))
 #define __builtin_amdgcn_s_sleep(n) ::simt::yield()
 #define __builtin_amdgcn_wave_barrier() ((void)::simt::wave_op(::simt::OP_SYNC, 0, 0))
 inline unsigned simt_alignbyte(unsigned hi, unsigned lo, unsigned r) {
@@ -122,15 +131,51 @@ inline unsigned simt_alignbyte(unsigned hi, unsigned lo, unsigned r) {
 #define __HIP_MEMORY_SCOPE_WORKGROUP 3
 #define __HIP_MEMORY_SCOPE_AGENT 4
 #define __HIP_MEMORY_SCOPE_SYSTEM 5
-#define __hip_atomic_load(p, o, s) __atomic_load_n((p), (o))
-#define __hip_atomic_store(p, v, o, s) __atomic_store_n((p), (v), (o))
-#define __hip_atomic_fetch_add(p, v, o, s) __atomic_fetch_add((p), (v), (o))
-#define __hip_atomic_compare_exchange_strong(p, e, d, so, fo, s) \
-    __atomic_compare_exchange_n((p), (e), (d), false, (so), (fo))
+// Scoped atomics: the access itself (traced like any load or store, which
+// keeps the wave's lockstep) is marked atomic for the checker, which then
+// applies the operation's synchronisation.  Nothing yields between the
+// access and the report, so the two are one step of the schedule.
+template <typename T>
+inline T simt_atomic_load(const T* p, int o, int s) {
+    ::simt::cur->in_atomic = true;
+    const T v = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    ::simt::cur->in_atomic = false;
+    ::simt::race_atomic(p, ::simt::RACE_LOAD, o, s);
+    return v;
+}
 template <typename T, typename U>
-inline T atomicAdd(T* p, U v) { return __atomic_fetch_add(p, (T)v, __ATOMIC_RELAXED); }
+inline void simt_atomic_store(T* p, U v, int o, int s) {
+    ::simt::cur->in_atomic = true;
+    __atomic_store_n(p, (T)v, __ATOMIC_SEQ_CST);
+    ::simt::cur->in_atomic = false;
+    ::simt::race_atomic(p, ::simt::RACE_STORE, o, s);
+}
 template <typename T, typename U>
-inline T atomicSub(T* p, U v) { return __atomic_fetch_sub(p, (T)v, __ATOMIC_RELAXED); }
+inline T simt_atomic_fetch_add(T* p, U v, int o, int s) {
+    const T r = __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST);
+    ::simt::race_atomic(p, ::simt::RACE_RMW, o, s);
+    return r;
+}
+template <typename T, typename U>
+inline bool simt_atomic_cas(T* p, T* e, U d, int so, int fo, int s) {
+    ::simt::cur->in_atomic = true;  // *e is the caller's own memory
+    const bool ok = __atomic_compare_exchange_n(p, e, (T)d, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    ::simt::cur->in_atomic = false;
+    ::simt::race_atomic(p, ::simt::RACE_RMW, ok ? so : fo, s);
+    return ok;
+}
+#define __hip_atomic_load(p, o, s) simt_atomic_load((p), (o), (s))
+#define __hip_atomic_store(p, v, o, s) simt_atomic_store((p), (v), (o), (s))
+#define __hip_atomic_fetch_add(p, v, o, s) simt_atomic_fetch_add((p), (v), (o), (s))
+#define __hip_atomic_compare_exchange_strong(p, e, d, so, fo, s) simt_atomic_cas((p), (e), (d), (so), (fo), (s))
+template <typename T, typename U>
+inline T atomicAdd(T* p, U v) { return simt_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T, typename U>
+inline T atomicSub(T* p, U v) {
+    const T r = __atomic_fetch_sub(p, (T)v, __ATOMIC_SEQ_CST);
+    ::simt::race_atomic(p, ::simt::RACE_RMW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return r;
+}
 
 inline float __uint_as_float(unsigned u) { return simt_from<float>(u); }
 inline unsigned __float_as_uint(float f) { return (unsigned)simt_bits(f); }

@@ -5,6 +5,8 @@
 //   run_coll abort P THREADS                   (rank 1 aborts; the others keep
 //                                               calling collectives)
 //   run_coll wire FMT P N THREADS              (compressed-wire Allreduce)
+//   run_coll race_selftest ORDERED             (the happens-before checker
+//                                               on a planted race, SIMT_RACE=1)
 #include "progs/registry.hpp"
 
 #include <cstdint>
@@ -37,6 +39,30 @@ GPUMPI_PROGRAM(sim_abort_collectives) {
             MPI_Bcast(buf, P, MPI_INT, 0, MPI_COMM_WORLD);
             MPI_Alltoall(buf, 1, MPI_INT, buf + 2 * P, 1, MPI_INT, MPI_COMM_WORLD);
         }
+    }
+    MPI_Finalize();
+    return 0;
+}
+
+// Self-test of the happens-before checker: rank 0 writes a word of a
+// shared buffer that rank 1 then reads, with (args[2] = 1) or without
+// (args[2] = 0) an MPI_Barrier between the two; and rank 1 overwrites a word
+// rank 0 read, with or without the barrier.  Only the variant without the
+// barrier may be reported.
+GPUMPI_PROGRAM(sim_race_selftest) {
+    MPI_Init(&argc, &argv);
+    int rank;
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    volatile int* buf = prog_ptr<int>(1);
+    const bool ordered = prog_int(2) != 0;
+    if (rank == 0) {
+        buf[0] = 42;
+        buf[8] += 1;
+    }
+    if (ordered) MPI_Barrier(MPI_COMM_WORLD);
+    if (rank == 1) {
+        buf[4] = buf[0];
+        buf[8] = 7;
     }
     MPI_Finalize();
     return 0;
@@ -189,6 +215,16 @@ static int wire(int wire_fmt, int P, int n, int threads) {
     return bad == 0 && rel < (wire_fmt == 1 ? 1e-2 : 6e-2) ? 0 : 1;
 }
 
+static int race_selftest(int ordered) {
+    int* buf = nullptr;
+    if (gpumpi_device_alloc((void**)&buf, 256)) return 4;
+    std::memset(buf, 0, 256);
+    std::vector<uint64_t> words(64, 0);
+    words[1] = (uint64_t)(uintptr_t)buf;
+    words[2] = (uint64_t)ordered;
+    return launch("sim_race_selftest", 2, 64, words);
+}
+
 static int abort_case(int P, int threads) {
     std::vector<uint64_t> words(64, 0);
     int err = 0;
@@ -201,6 +237,7 @@ int main(int argc, char** argv) {
     if (argc >= 4 && !std::strcmp(argv[1], "abort")) return abort_case(std::atoi(argv[2]), std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "alltoallv")) return alltoallv(argv[2], std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "blocks")) return blocks(std::atoi(argv[2]), std::atoi(argv[3]));
+    if (argc >= 3 && !std::strcmp(argv[1], "race_selftest")) return race_selftest(std::atoi(argv[2]));
     if (argc >= 6 && !std::strcmp(argv[1], "wire"))
         return wire(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     std::fprintf(stderr, "usage: run_coll alltoallv COUNTS THREADS | blocks KIND THREADS | abort P THREADS\n");

@@ -4,8 +4,12 @@
 #include <sanitizer/asan_interface.h>
 #include <sys/mman.h>
 
+#include <sanitizer/common_interface_defs.h>
+
 #include <chrono>
 #include <map>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 // Dynamic LDS of the running workgroup (device.hpp declares the symbol as
@@ -200,6 +204,159 @@ void mem_step(const void* addr) {
     wait_change(&w.step_gen, g);
 }
 
+// ---------------------------------------------------------------------------
+// Happens-before checker (SIMT_RACE=1).
+//
+// Threads are ranks (workgroups): the waves of one rank synchronise through
+// LDS and are not checked against each other.  Each rank carries a vector
+// clock.  Synchronisation follows the scoped-atomic model of the device
+// code, agent or system scope only (a workgroup-scope fence or atomic
+// orders nothing between ranks):
+//   release fence        rel[A] = VC[A]; VC[A][A]++
+//   atomic store/RMW     sync[x] = rel[A] (store) or sync[x] |= rel[A] (RMW),
+//                        after a release-fence step of its own if the
+//                        operation is itself release
+//   atomic load/RMW      acquire: VC[B] |= sync[x]; relaxed: pend[B] |= sync[x]
+//   acquire fence        VC[B] |= pend[B]
+// Every plain load and store of global memory (4-byte words; 1- and 2-byte
+// accesses by byte) keeps the last write epoch and the read epochs since
+// then; an access that conflicts with another rank's access it is not
+// ordered after is reported with both sites.  A launch starts from a clean
+// state: the host's writes before it, and everything of the launch before,
+// are ordered by the launch itself.
+// ---------------------------------------------------------------------------
+static bool g_race = false;
+static int g_nranks = 0;
+static std::vector<std::vector<uint32_t>> g_vc, g_rel, g_pend;
+struct RaceEpoch {
+    int32_t r = -1;
+    uint32_t c = 0;
+    const void* pc = nullptr;
+};
+struct RaceShadow {
+    RaceEpoch w;
+    RaceEpoch rd;                       // the one read epoch, or
+    std::vector<RaceEpoch>* rv = nullptr;  // per-rank read epochs once reads are concurrent
+};
+static std::unordered_map<uintptr_t, RaceShadow> g_shadow;
+static std::unordered_map<uintptr_t, std::vector<uint32_t>> g_sync;
+static std::unordered_set<uint64_t> g_race_seen;
+static uint64_t g_race_reports = 0;
+
+static void race_reset(int nranks) {
+    const char* e = std::getenv("SIMT_RACE");
+    g_race = e && e[0] == '1';
+    if (!g_race) return;
+    g_nranks = nranks;
+    g_vc.assign(nranks, std::vector<uint32_t>(nranks, 0));
+    g_rel = g_vc;
+    g_pend = g_vc;
+    for (int r = 0; r < nranks; r++) g_vc[r][r] = 1;
+    for (auto& kv : g_shadow) delete kv.second.rv;
+    g_shadow.clear();
+    g_sync.clear();
+}
+static void join(std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
+    for (size_t i = 0; i < a.size() && i < b.size(); i++)
+        if (b[i] > a[i]) a[i] = b[i];
+}
+static int race_rank() {
+    if (!g_race || !cur->block || cur->in_atomic) return -1;
+    return (int)cur->bid.x;
+}
+static void race_report(const char* what, uintptr_t key, const RaceEpoch& prev, int me, const void* pc) {
+    const uint64_t sig = (uint64_t)(uintptr_t)prev.pc * 1000003u ^ (uint64_t)(uintptr_t)pc;
+    g_race_reports++;
+    if (!g_race_seen.insert(sig).second || g_race_seen.size() > 40) return;
+    char a[512], b[512];
+    __sanitizer_symbolize_pc(const_cast<void*>(pc), "%F %L", a, sizeof a);
+    __sanitizer_symbolize_pc(const_cast<void*>(prev.pc), "%F %L", b, sizeof b);
+    std::fprintf(stderr,
+                 "simt race: %s at %#lx: rank %d (clock of rank %d: %u) is not ordered after rank %d's access "
+                 "(epoch %u)\n  this access: %s\n  other access: %s\n",
+                 what, (unsigned long)(key & ~(1ull << 63)), me, prev.r, g_vc[me][prev.r], prev.r, prev.c, a, b);
+}
+static bool race_checked(const void* p) {
+    Fiber* f = cur;
+    const uintptr_t a = (uintptr_t)p;
+    if (a >= (uintptr_t)f->block->lds && a < (uintptr_t)f->block->lds + f->block->lds_bytes) return false;
+    if (a >= (uintptr_t)f->stack_lo && a < (uintptr_t)f->stack_lo + f->stack_size) return false;
+    return true;
+}
+static void race_word(uintptr_t key, bool store, int me, const void* pc) {
+    RaceShadow& sh = g_shadow[key];
+    const std::vector<uint32_t>& vc = g_vc[me];
+    if (sh.w.r >= 0 && sh.w.r != me && sh.w.c > vc[sh.w.r])
+        race_report(store ? "write after unordered write" : "read of an unordered write", key, sh.w, me, pc);
+    if (store) {
+        if (sh.rv) {
+            for (const RaceEpoch& e : *sh.rv)
+                if (e.r >= 0 && e.r != me && e.c > vc[e.r]) {
+                    race_report("write after an unordered read", key, e, me, pc);
+                    break;
+                }
+            delete sh.rv;
+            sh.rv = nullptr;
+        } else if (sh.rd.r >= 0 && sh.rd.r != me && sh.rd.c > vc[sh.rd.r]) {
+            race_report("write after an unordered read", key, sh.rd, me, pc);
+        }
+        sh.rd = RaceEpoch();
+        sh.w = RaceEpoch{me, vc[me], pc};
+        return;
+    }
+    const RaceEpoch now{me, vc[me], pc};
+    if (sh.rv) {
+        if ((size_t)me < sh.rv->size()) (*sh.rv)[me] = now;
+    } else if (sh.rd.r < 0 || sh.rd.r == me || sh.rd.c <= vc[sh.rd.r]) {
+        sh.rd = now;
+    } else {
+        sh.rv = new std::vector<RaceEpoch>(g_nranks);
+        (*sh.rv)[sh.rd.r] = sh.rd;
+        (*sh.rv)[me] = now;
+    }
+}
+static void race_access(const void* p, int n, bool store, const void* pc) {
+    const int me = race_rank();
+    if (me < 0 || !race_checked(p)) return;
+    const uintptr_t a = (uintptr_t)p;
+    if (n < 4) {
+        for (int i = 0; i < n; i++) race_word((a + i) | (1ull << 63), store, me, pc);
+        return;
+    }
+    for (uintptr_t w = a & ~(uintptr_t)3; w < a + n; w += 4) race_word(w, store, me, pc);
+}
+static bool is_release(int o) { return o == __ATOMIC_RELEASE || o == __ATOMIC_ACQ_REL || o == __ATOMIC_SEQ_CST; }
+static bool is_acquire(int o) {
+    return o == __ATOMIC_ACQUIRE || o == __ATOMIC_ACQ_REL || o == __ATOMIC_SEQ_CST || o == __ATOMIC_CONSUME;
+}
+static void release_step(int me) {
+    g_rel[me] = g_vc[me];
+    g_vc[me][me]++;
+}
+void race_fence(int order, const char* scope) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (!g_race || !cur->block) return;
+    if (std::strcmp(scope, "agent") != 0 && scope[0] != 0) return;  // workgroup / wavefront
+    const int me = (int)cur->bid.x;
+    if (is_release(order)) release_step(me);
+    if (is_acquire(order)) join(g_vc[me], g_pend[me]);
+}
+void race_atomic(const void* p, int kind, int order, int scope) {
+    if (!g_race || !cur->block || scope < 4) return;  // below agent scope: rank-local
+    const int me = (int)cur->bid.x;
+    const uintptr_t key = (uintptr_t)p;
+    if (kind != RACE_STORE) {
+        auto it = g_sync.find(key);
+        if (it != g_sync.end()) join(is_acquire(order) ? g_vc[me] : g_pend[me], it->second);
+    }
+    if (kind != RACE_LOAD) {
+        if (is_release(order)) release_step(me);
+        std::vector<uint32_t>& sy = g_sync[key];
+        if (kind == RACE_STORE || sy.empty()) sy = g_rel[me];
+        else join(sy, g_rel[me]);
+    }
+}
+
 __attribute__((noinline)) uint64_t wave_op(int op, uint64_t v, int arg) {
     Fiber* f = cur;
     if (!f->wave) fail("wave operation outside a rank kernel");
@@ -307,6 +464,7 @@ static int launch(const void* k, dim3 grid, dim3 block, void** args, size_t lds)
             f.sp = top;
         }
     }
+    race_reset((int)nb);
     size_t live = fibers.size();
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t passes = 0;
@@ -358,15 +516,27 @@ static int launch(const void* k, dim3 grid, dim3 block, void** args, size_t lds)
     munmap(stacks, S * fibers.size());
     for (Block& B : blocks) std::free(B.lds);
     cur = &main_fiber;
+    if (g_race) {
+        std::fprintf(stderr, "simt race check: %llu report(s) in this launch (%zu distinct site pairs)\n",
+                     (unsigned long long)g_race_reports, g_race_seen.size());
+        g_race_reports = 0;
+        g_race_seen.clear();
+    }
     return hipSuccess;
 }
 
 }  // namespace simt
 
 extern "C" {
-#define SIMT_STEP(n, T) \
-    void __sanitizer_cov_load##n(T* p) { simt::mem_step(p); } \
-    void __sanitizer_cov_store##n(T* p) { simt::mem_step(p); }
+#define SIMT_STEP(n, T)                                                              \
+    void __sanitizer_cov_load##n(T* p) {                                             \
+        simt::mem_step(p);                                                           \
+        if (simt::g_race) simt::race_access(p, n, false, __builtin_return_address(0)); \
+    }                                                                                \
+    void __sanitizer_cov_store##n(T* p) {                                            \
+        simt::mem_step(p);                                                           \
+        if (simt::g_race) simt::race_access(p, n, true, __builtin_return_address(0));  \
+    }
 SIMT_STEP(1, uint8_t)
 SIMT_STEP(2, uint16_t)
 SIMT_STEP(4, uint32_t)

@@ -4,7 +4,8 @@ under AddressSanitizer and UBSan.  These run the launcher and the programs
 end to end without a GPU: the batched uneven Allreduce (exact, BF16 and FP8
 wires), the compressed-wire Allreduce, the uneven Alltoallv and the block collectives at the shapes of the
 GPU tests, and a launch in which one rank aborts while the others keep
-calling collectives."""
+calling collectives.  With SIMT_RACE=1 the simulation also checks every
+cross-rank access for happens-before ordering (simt.cpp)."""
 import os
 import subprocess
 
@@ -85,6 +86,48 @@ def test_sim_wire_allreduce(sim, wire, P, n):
     # out-of-bounds access under AddressSanitizer.
     rc, out = _run([os.path.join(sim, "run_coll"), "wire", str(wire), str(P), str(n), "256"])
     assert rc == 0 and f"wire {wire} P={P} n={n}: rank mismatches 0" in out, out[-3000:]
+
+
+def _run_race(args, timeout=900):
+    env = dict(ENV, SIMT_RACE="1")
+    p = subprocess.run(args, capture_output=True, text=True, env=env, timeout=timeout)
+    return p.returncode, p.stdout + p.stderr
+
+
+def test_sim_race_checker_flags_a_planted_race(sim):
+    # The happens-before checker itself: a write by rank 0 that rank 1 reads
+    # with no synchronisation between them is reported with both sites; the
+    # same accesses separated by MPI_Barrier are not.
+    rc, out = _run_race([os.path.join(sim, "run_coll"), "race_selftest", "0"])
+    assert rc == 0, out[-3000:]
+    assert "simt race: read of an unordered write" in out and "run_coll.cpp" in out, out[-3000:]
+    assert "simt race: write after an unordered read" in out
+    rc, out = _run_race([os.path.join(sim, "run_coll"), "race_selftest", "1"])
+    assert rc == 0 and "simt race:" not in out and "0 report(s)" in out, out[-3000:]
+
+
+@pytest.mark.parametrize("case", [["blocks", str(k), "128"] for k in range(7)] +
+                         [["wire", "1", "4", "1000", "256"], ["wire", "2", "9", "2000", "256"], ["abort", "8", "128"]],
+                         ids=lambda c: "-".join(c))
+def test_sim_collectives_have_no_unordered_accesses(sim, case):
+    # Every plain access a rank makes to another rank's data (table words,
+    # member buffers, the communicator's shared state) is ordered by the
+    # agent-scope release/acquire of a barrier or a channel word.
+    rc, out = _run_race([os.path.join(sim, "run_coll")] + case)
+    assert rc == 0, out[-3000:]
+    assert "simt race:" not in out and "race check: 0 report(s)" in out, out[-3000:]
+
+
+def test_sim_alltoallv_has_no_unordered_accesses(sim, tmp_path):
+    P = 4
+    rng = np.random.default_rng(P)
+    counts = rng.integers(0, 3000, (P, P)).astype(np.int32)
+    counts[rng.random((P, P)) < 0.1] = 0
+    f = tmp_path / "counts.txt"
+    f.write_text(f"{P}\n" + " ".join(map(str, counts.ravel())) + "\n")
+    rc, out = _run_race([os.path.join(sim, "run_coll"), "alltoallv", str(f), "256"])
+    assert rc == 0 and "alltoallv mismatches: 0" in out, out[-3000:]
+    assert "simt race:" not in out and "race check: 0 report(s)" in out, out[-3000:]
 
 
 def test_sim_abort_leaves_unpublished_tables_alone(sim):
