@@ -58,7 +58,8 @@ def _compile(src, obj, force):
 def build(jobs=8, force=False, verbose=True):
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
-    host_src = [os.path.join(CSRC, "host", "launcher.cpp"), os.path.join(CSRC, "host", "noalloc.hip")]
+    host_src = [os.path.join(CSRC, "host", "launcher.cpp"), os.path.join(CSRC, "host", "service.cpp"),
+                os.path.join(CSRC, "host", "noalloc.hip")]
     prog_src = sorted(glob.glob(os.path.join(CSRC, "progs", "*.hip"))) + \
         [os.path.join(CSRC, "progs", "registry.cpp")]
     units = [(s, os.path.join(OBJ, os.path.basename(s) + ".o")) for s in host_src + prog_src]

@@ -662,4 +662,5 @@ __device__ inline int MPI_NULL_COPY_FN(MPI_Comm, int, void*, void*, void*, int* 
 __device__ inline int MPI_NULL_DELETE_FN(MPI_Comm, int, void*, void*) { return MPI_SUCCESS; }
 
 #include "wire.hpp"
+#include "io.hpp"
 #include "kernel.hpp"

@@ -109,6 +109,46 @@ struct ErrorRecord {
     char msg[112];
 };
 
+// Device -> host service slot of one rank, in pinned coherent host memory:
+// file I/O (MPI_File_*) and formatted text output, carried out by the host
+// while the launch runs (the reference's delegate_to_host ring,
+// gpu_libs/gpu_libc/cuda_mpi.cuh:392-435 and gpu_main/libc_processor.cu:15-84).
+// A rank has at most one request outstanding: its program waits for each.
+//   device: fill op/args/data, release, state = POSTED, wait for DONE
+//   host:   acquire POSTED, carry out, write results, release, state = DONE
+//   device: read results, state = IDLE
+constexpr int kSvcSlotBytes = 16384;
+constexpr int kSvcDataBytes = kSvcSlotBytes - 64;
+enum SvcState : uint32_t { SVC_IDLE = 0, SVC_POSTED = 1, SVC_DONE = 2 };
+enum SvcOp : int32_t {
+    SVC_OPEN = 1,    // data: path; i[0] amode -> l[0] handle, l[1] size, i[1] error
+    SVC_CLOSE = 2,   // l[0] handle, i[0] delete on close -> i[1] error
+    SVC_READ = 3,    // l[0] handle, l[1] byte offset, i[0] bytes -> data, i[1] bytes read, i[2] error
+    SVC_WRITE = 4,   // l[0] handle, l[1] byte offset, i[0] bytes, data -> i[1] bytes written, i[2] error
+    SVC_SIZE = 5,    // l[0] handle -> l[1] size, i[1] error
+    SVC_DELETE = 6,  // data: path -> i[1] error
+    SVC_SYNC = 7,    // l[0] handle -> i[1] error
+    SVC_FORMAT = 8,  // data: format, then argument records; i[0] stream (0: back into data,
+                     // 1 stdout, 2 stderr), i[1] argument count, i[2] offset of the records,
+                     // i[3] capacity for stream 0 -> i[4] formatted length
+};
+// One printf argument as the device packs it (strings follow inline, padded
+// to 8 bytes).
+enum SvcArgKind : int32_t { SVC_ARG_INT = 1, SVC_ARG_UINT = 2, SVC_ARG_DOUBLE = 3, SVC_ARG_STR = 4, SVC_ARG_PTR = 5 };
+struct SvcArg {
+    int32_t kind;
+    int32_t len;  // string bytes that follow (without the NUL)
+    uint64_t bits;
+};
+struct alignas(64) SvcSlot {
+    uint32_t state;
+    int32_t op;
+    int32_t i[6];
+    int64_t l[4];
+    uint8_t data[kSvcDataBytes];
+};
+static_assert(sizeof(SvcSlot) == kSvcSlotBytes, "service slot layout");
+
 struct World {
     Config cfg;
     uint8_t* slots;        // [P*P*Q] slots of slot_bytes, index (dst*P+src)*Q+k
@@ -138,6 +178,9 @@ struct World {
     // Device copy of the abort flag: set (release) by a rank that raises an
     // MPI error, read by every barrier after it completes.
     uint32_t* dev_abort;
+    // [P] service slots (pinned host), or null when no host serves this
+    // launch (then file I/O fails with MPI_ERR_IO and text output is dropped).
+    SvcSlot* svc;
 };
 
 // Per-rank save block: stamp, validity magic, then RxChan[P] and TxChan[P]

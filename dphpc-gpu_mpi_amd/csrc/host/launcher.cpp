@@ -224,6 +224,11 @@ int init_world(const Config& cfg, void* user_args) {
     W.abort_word[1] = 0;  // set by a rank that exits with undelivered messages
     std::memset((void*)W.err, 0, sizeof(ErrorRecord));
     HIPOK(hipMemsetAsync(W.dev_abort, 0, 256, s));
+    SvcSlot* svc = nullptr;
+    {
+        const int rc = gpumpi::host::svc_prepare(cfg.nranks, &svc);
+        if (rc) return rc;
+    }
 
     World hw{};
     hw.cfg = cfg;
@@ -247,6 +252,7 @@ int init_world(const Config& cfg, void* user_args) {
     hw.members_hbm = W.members_hbm;
     hw.dev_abort = W.dev_abort;
     hw.save_bytes = W.save_bytes;
+    hw.svc = svc;
     HIPOK(hipMemcpyAsync(W.dworld, &hw, sizeof hw, hipMemcpyHostToDevice, s));
     HIPOK(hipStreamSynchronize(s));
     return GPUMPI_OK;
@@ -417,12 +423,15 @@ int gpumpi_max_ranks(const gpumpi_config* cfg, const void* kernel) {
 
 namespace {
 
-// Waits for ev1 with the watchdog; returns a GPUMPI status.
+// Waits for ev1 with the watchdog, serving the ranks' host requests
+// (MPI_File_*, printf) meanwhile; returns a GPUMPI status.
 int wait_launch(const gpumpi_config* cfg_in, gpumpi_result& R) {
     WorldAlloc& W = g_world;
     const auto t0 = std::chrono::steady_clock::now();
     bool fired = false;
+    int busy = 0;  // polls left without sleeping after a served request
     for (;;) {
+        if (gpumpi::host::svc_poll() > 0) busy = 2000;
         hipError_t q = hipEventQuery(W.ev1);
         if (q == hipSuccess) break;
         if (q != hipErrorNotReady) {
@@ -435,7 +444,8 @@ int wait_launch(const gpumpi_config* cfg_in, gpumpi_result& R) {
             fired = true;
             R.timed_out = 1;
         }
-        std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (busy > 0) busy--;
+        else std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
     float ms = 0.f;
     HIPOK(hipEventElapsedTime(&ms, W.ev0, W.ev1));

@@ -23,6 +23,10 @@ extern thread_local char g_err[512];
 void set_err(const char* fmt, ...);
 // Checked launch configuration and the rank workgroup's LDS bytes.
 int build_config(const gpumpi_config* in, rt::Config* out, int* lds_bytes);
+// Host service channel (service.cpp): slots for `nranks` ranks, all idle,
+// served by svc_poll until the next svc_prepare.
+int svc_prepare(int nranks, rt::SvcSlot** out);
+int svc_poll();
 
 }  // namespace host
 }  // namespace gpumpi

@@ -39,7 +39,7 @@ CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpu
                 "gpumpi_main",
                 "gpumpi_device_alloc", "gpumpi_device_free", "gpumpi_memcpy_htod",
                 "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error",
-                "gpumpi_reset_world", "gpumpi_release_world"]
+                "gpumpi_reset_world", "gpumpi_release_world", "gpumpi_service_poll"]
 PROGS_SYMBOLS = ["gpumpi_prog_lookup", "gpumpi_prog_count", "gpumpi_prog_name",
                  "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal",
                  "gpumpi_copy_calibration"]

@@ -117,6 +117,14 @@ int gpumpi_memcpy_htod(void* dst, const void* src, size_t bytes);
 int gpumpi_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int gpumpi_memset(void* dst, int value, size_t bytes);
 
+/* Carry out the pending requests of the running launch's ranks on the host
+ * service channel (MPI_File_*, the printf family) once; returns how many.
+ * gpumpi_run and gpumpi_run_graph do this themselves while they wait; a host
+ * that waits for a launch by other means calls it in its loop.  Replaces the
+ * reference's host polling of its device-to-host queue
+ * (gpu_libs/gpu_libc/cuda_mpi.cuh:414-427, gpu_main/main.cu:260-300). */
+int gpumpi_service_poll(void);
+
 /* Start the next launch from fresh channel state (by default channel
  * cursors carry over between clean launches of the same world shape). */
 int gpumpi_reset_world(void);

@@ -10,6 +10,8 @@
  *   gpu_libs/gpu_mpi/mpi_operators_list.cuh:4-16 (operator order)
  *   gpu_libs/gpu_mpi/communicator.cuh:6-49   (MPI_Comm and its calls)
  *   gpu_libs/gpu_mpi/group.cuh:4-31          (MPI_Group and its calls)
+ *   gpu_libs/gpu_mpi/io.cuh:13-153           (MPI_File_*: amode, error and seek
+ *                                             constants, MPI_File by value)
  *
  * Execution model: every MPI rank is one workgroup of a single persistent
  * kernel.  Wave 0 of the workgroup runs the rank's program (all 64 lanes
@@ -26,7 +28,12 @@
  *   - MPI_Wait(MPI_REQUEST_NULL) and MPI_Waitall(MPI_STATUSES_IGNORE) are legal;
  *   - MPI_LXOR is the logical exclusive or (the reference used a != b);
  *   - Reductions of every datatype are a deterministic rank-order fold
- *     (the reference's WORLD double sum used atomicAdd, mpi.cu:148-179).
+ *     (the reference's WORLD double sum used atomicAdd, mpi.cu:148-179);
+ *   - MPI-IO: MPI_Offset is 64-bit; offsets and positions count etypes of the
+ *     view, as MPI specifies (the reference counted bytes); MPI_File_open
+ *     does not truncate an existing file and MPI_MODE_EXCL fails only when
+ *     the file exists; MPI_File_delete is local; read_at/write_at return the
+ *     element count like read/write.
  */
 #ifndef GPUMPI_MPI_H
 #define GPUMPI_MPI_H
@@ -59,6 +66,41 @@
 #define MPI_ERR_TAG 29
 #define MPI_ERR_BUFFER 30
 #define MPI_ERR_INTERN 31
+
+/* MPI-IO error classes, io.cuh:28-44 (their values collide with
+ * MPI_ERR_OTHER and each other's neighbours there too). */
+#define MPI_ERR_FILE 1
+#define MPI_ERR_NOT_SAME 2
+#define MPI_ERR_AMODE 3
+#define MPI_ERR_UNSUPPORTED_DATAREP 4
+#define MPI_ERR_UNSUPPORTED_OPERATION 5
+#define MPI_ERR_NO_SUCH_FILE 6
+#define MPI_ERR_FILE_EXISTS 7
+#define MPI_ERR_BAD_FILE 8
+#define MPI_ERR_ACCESS 9
+#define MPI_ERR_NO_SPACE 10
+#define MPI_ERR_QUOTA 11
+#define MPI_ERR_READ_ONLY 12
+#define MPI_ERR_FILE_IN_USE 13
+#define MPI_ERR_DUP_DATAREP 14
+#define MPI_ERR_CONVERSION 15
+#define MPI_ERR_IO 16
+
+/* amode bits, io.cuh:13-21 */
+#define MPI_MODE_RDONLY 1
+#define MPI_MODE_RDWR 2
+#define MPI_MODE_WRONLY 4
+#define MPI_MODE_CREATE 8
+#define MPI_MODE_EXCL 16
+#define MPI_MODE_DELETE_ON_CLOSE 32
+#define MPI_MODE_UNIQUE_OPEN 64
+#define MPI_MODE_SEQUENTIAL 128
+#define MPI_MODE_APPEND 256
+/* io.cuh:50-52, 76 */
+#define MPI_SEEK_SET 1
+#define MPI_SEEK_CUR 2
+#define MPI_SEEK_END 3
+#define MPI_DISPLACEMENT_CURRENT (-1)
 
 #define MPI_THREAD_SINGLE 1
 #define MPI_THREAD_FUNNELED 2
@@ -234,6 +276,52 @@ __device__ int MPI_Keyval_create(MPI_Copy_function* copy_fn, MPI_Delete_function
 __device__ int MPI_NULL_COPY_FN(MPI_Comm oldcomm, int keyval, void* extra_state,
                                 void* attribute_val_in, void* attribute_val_out, int* flag);
 __device__ int MPI_NULL_DELETE_FN(MPI_Comm comm, int keyval, void* attribute_val, void* extra_state);
+
+/* ---- MPI-IO: io.cuh:56-153 --------------------------------------------- */
+typedef long long MPI_Offset;
+typedef struct MPI_Info_s {
+    int gpumpi_unused;
+} MPI_Info;
+#define MPI_INFO_NULL (MPI_Info{0})
+struct gpumpi_file_state;
+/* By value, as io.cuh:93-101: `file` is the host's file handle (non-null
+ * while open), `st` the positions and views shared by the members. */
+typedef struct MPI_File_s {
+    MPI_Comm comm;
+    void* file;
+    int amode;
+    struct gpumpi_file_state* st;
+} MPI_File;
+#define MPI_FILE_NULL (MPI_File{MPI_COMM_NULL, nullptr, 0, nullptr})
+__device__ inline bool operator==(const MPI_File& a, const MPI_File& b) { return a.file == b.file && a.st == b.st; }
+__device__ inline bool operator!=(const MPI_File& a, const MPI_File& b) { return !(a == b); }
+
+__device__ int MPI_File_open(MPI_Comm comm, const char* filename, int amode, MPI_Info info, MPI_File* fh);
+__device__ int MPI_File_close(MPI_File* fh);
+__device__ int MPI_File_delete(const char* filename, MPI_Info info);
+__device__ int MPI_File_get_size(MPI_File fh, MPI_Offset* size);
+__device__ int MPI_File_get_amode(MPI_File fh, int* amode);
+__device__ int MPI_File_sync(MPI_File fh);
+__device__ int MPI_File_set_view(MPI_File fh, MPI_Offset disp, MPI_Datatype etype, MPI_Datatype filetype,
+                                 const char* datarep, MPI_Info info);
+__device__ int MPI_File_get_view(MPI_File fh, MPI_Offset* disp, MPI_Datatype* etype, MPI_Datatype* filetype,
+                                 char* datarep);
+__device__ int MPI_File_read(MPI_File fh, void* buf, int count, MPI_Datatype datatype, MPI_Status* status);
+__device__ int MPI_File_write(MPI_File fh, const void* buf, int count, MPI_Datatype datatype, MPI_Status* status);
+__device__ int MPI_File_read_at(MPI_File fh, MPI_Offset offset, void* buf, int count, MPI_Datatype datatype,
+                                MPI_Status* status);
+__device__ int MPI_File_write_at(MPI_File fh, MPI_Offset offset, const void* buf, int count,
+                                 MPI_Datatype datatype, MPI_Status* status);
+__device__ int MPI_File_read_all(MPI_File fh, void* buf, int count, MPI_Datatype datatype, MPI_Status* status);
+__device__ int MPI_File_write_all(MPI_File fh, const void* buf, int count, MPI_Datatype datatype,
+                                  MPI_Status* status);
+__device__ int MPI_File_read_shared(MPI_File fh, void* buf, int count, MPI_Datatype datatype, MPI_Status* status);
+__device__ int MPI_File_write_shared(MPI_File fh, const void* buf, int count, MPI_Datatype datatype,
+                                     MPI_Status* status);
+__device__ int MPI_File_seek(MPI_File fh, MPI_Offset offset, int whence);
+__device__ int MPI_File_get_position(MPI_File fh, MPI_Offset* offset);
+__device__ int MPI_File_seek_shared(MPI_File fh, MPI_Offset offset, int whence);
+__device__ int MPI_File_get_position_shared(MPI_File fh, MPI_Offset* offset);
 
 /* ---- groups: group.cuh:21-31 ------------------------------------------- */
 __device__ int MPI_Group_size(MPI_Group group, int* size);

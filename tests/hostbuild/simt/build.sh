@@ -13,7 +13,7 @@ FLAGS="-std=c++17 -g -O1 -fno-omit-frame-pointer -ffp-contract=off -fsanitize=ad
 COV="-fsanitize-coverage=inline-8bit-counters,trace-loads,trace-stores"
 OBJ="$OUT/obj"
 mkdir -p "$OBJ"
-common="$ROOT/dphpc-gpu_mpi_amd/csrc/host/launcher.cpp $ROOT/dphpc-gpu_mpi_amd/csrc/progs/registry.cpp"
+common="$ROOT/dphpc-gpu_mpi_amd/csrc/host/launcher.cpp $ROOT/dphpc-gpu_mpi_amd/csrc/host/service.cpp $ROOT/dphpc-gpu_mpi_amd/csrc/progs/registry.cpp"
 if [ -n "$SIMT_PROGS" ]; then
   for p in $SIMT_PROGS; do common="$common $ROOT/dphpc-gpu_mpi_amd/csrc/progs/$p"; done
 else

@@ -427,6 +427,12 @@ void plain_thread(unsigned bx, unsigned by, unsigned t) {
 }
 void plain_end() { cur = &main_fiber; }
 
+// The launcher's service-channel poll (host/service.cpp), when linked: the
+// simulated launch runs inside hipLaunchCooperativeKernel, so the scheduler
+// serves the ranks' host requests itself, as the launcher's wait loop does
+// on the GPU.
+extern "C" int gpumpi_service_poll(void) __attribute__((weak));
+
 static int launch(const void* k, dim3 grid, dim3 block, void** args, size_t lds) {
     if (block.x % 64) return hipErrorInvalidValue;
     grid_dim = grid;
@@ -484,6 +490,12 @@ static int launch(const void* k, dim3 grid, dim3 block, void** args, size_t lds)
         }
         passes++;
         g_pass = passes;
+        if (gpumpi_service_poll && (passes & 15) == 0) {
+            Fiber* keep = cur;
+            cur = &main_fiber;
+            gpumpi_service_poll();
+            cur = keep;
+        }
         if ((passes & 255) == 0) {
             const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             bool stuck = el > limit;

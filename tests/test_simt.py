@@ -20,9 +20,12 @@ ENV = dict(os.environ, ASAN_OPTIONS="detect_odr_violation=0:detect_leaks=0", SIM
 
 @pytest.fixture(scope="module")
 def sim():
-    env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip")
-    subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT, os.path.join(SIM, "run_coll.cpp"),
-                    os.path.join(SIM, "run_batched.cpp")], check=True, env=env, timeout=1200)
+    # One build of every simulated program and driver: build.sh compiles all
+    # translation units in parallel.
+    env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip")
+    subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] +
+                   [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp")],
+                   check=True, env=env, timeout=1200)
     return OUT
 
 
@@ -107,7 +110,7 @@ def test_sim_race_checker_flags_a_planted_race(sim):
 
 
 @pytest.mark.parametrize("case", [["blocks", str(k), "128"] for k in range(7)] +
-                         [["wire", "1", "4", "1000", "256"], ["wire", "2", "9", "2000", "256"], ["abort", "8", "128"]],
+                         [["wire", "1", "4", "1000", "256"], ["wire", "2", "9", "2000", "256"]],
                          ids=lambda c: "-".join(c))
 def test_sim_collectives_have_no_unordered_accesses(sim, case):
     # Every plain access a rank makes to another rank's data (table words,
@@ -134,18 +137,12 @@ def test_sim_abort_leaves_unpublished_tables_alone(sim):
     # Rank 1 aborts, then reaches MPI_Finalize; its barrier arrival completes
     # the others' Allreduce barrier.  They must see the abort and return
     # instead of folding from table entries rank 1 never published.
-    rc, out = _run([os.path.join(sim, "run_coll"), "abort", "8", "128"])
+    # Under the happens-before checker: no rank reads what the aborting
+    # rank never published, and nothing it did publish unordered.
+    rc, out = _run_race([os.path.join(sim, "run_coll"), "abort", "8", "128"])
     assert rc == 0, out[-3000:]
     assert "mpi_error=7" in out and "AddressSanitizer" not in out
-
-
-@pytest.fixture(scope="module")
-def sim_is():
-    out = os.path.join(HERE, "hostbuild", "_build", "simt_is")
-    env = dict(os.environ, SIMT_PROGS="npb_is.hip")
-    subprocess.run(["bash", os.path.join(SIM, "build.sh"), out, os.path.join(SIM, "run_is.cpp")], check=True,
-                   env=env, timeout=1200)
-    return out
+    assert "simt race:" not in out and "race check: 0 report(s)" in out, out[-3000:]
 
 
 def _keyhash(keys):
@@ -160,8 +157,30 @@ def _keyhash(keys):
 # per-wave sub-histograms and (where the key range fits) the ranking counts
 # in LDS.
 @pytest.mark.parametrize("cls,P,ulds", [("S", 4, 4096), ("S", 4, 20480), ("S", 16, 20480), ("W", 4, 36864)])
-def test_sim_npb_is_matches_oracle(sim_is, orc, cls, P, ulds):
+def test_sim_npb_is_matches_oracle(sim, orc, cls, P, ulds):
     res, keys, _ = orc.is_run(cls, P, want_keys=True)
-    rc, out = _run([os.path.join(sim_is, "run_is"), cls, str(P), "128", str(ulds)])
+    rc, out = _run([os.path.join(sim, "run_is"), cls, str(P), "128", str(ulds)])
     assert rc == 0, out[-3000:]
     assert (f"passed={res.passed_verification} total={keys.size} errors=0 keyhash={_keyhash(keys)}") in out, out[-2000:]
+
+
+@pytest.mark.parametrize("P,N", [(1, 5), (4, 10), (9, 300)])
+def test_sim_io_scenarios(sim, tmp_path, P, N):
+    # MPI_File_* and the printf family on the host service channel: the
+    # reference's IO test cases plus explicit offsets, the shared pointer,
+    # multi-slot transfers, append/exclusive opens and formatted text.
+    import io_expect
+    rc, out = _run([os.path.join(sim, "run_io"), str(P), str(tmp_path), str(N)])
+    assert rc == 0, out[-3000:]
+    recs, texts = {}, {}
+    for line in out.splitlines():
+        if line.startswith("rank "):
+            r, vals = line[5:].split(":", 1)
+            recs[int(r)] = [int(v) for v in vals.split()]
+        elif line.startswith("text "):
+            r, t = line[5:].split(": ", 1)
+            texts[int(r)] = t
+    io_expect.check([recs[r] for r in range(P)], [texts[r] for r in range(P)], P, N)
+    for r in range(P):
+        assert f"io_scenarios: rank {r} of {P}" in out
+    assert os.listdir(tmp_path) == []  # every file the program made is deleted
