@@ -3,6 +3,8 @@
   lib/libgpumpi.so        host C-ABI (include/gpumpi/host.h): launcher, watchdog
   lib/libgpumpi_progs.so  rank programs (tests, workloads, benchmarks) with a
                           by-name registry, linked against libgpumpi.so
+  lib/ring, lib/cpi       C MPI programs converted by converter.py and built by
+                          gpumpicc.py (cpi from the reference's source, when present)
 
 Usage: python dphpc-gpu_mpi_amd/build.py [-j N] [--force]
 """
@@ -12,13 +14,16 @@ import glob
 import os
 import subprocess
 import sys
+import time
 
 PKG = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, PKG)
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "lib")
 OBJ = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+REFERENCE = os.environ.get("GPUMPI_REFERENCE", "/root/reference")
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -81,13 +86,23 @@ def build(jobs=8, force=False, verbose=True):
     if force or _stale(progs, prog_objs + [core]):
         subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", progs] + prog_objs +
                        ["-L", LIB, "-lgpumpi", "-Wl,-rpath,$ORIGIN"], check=True)
-    # The converted-program example: a standalone executable with the
-    # reference's `---gpumpi` command line.
-    exe = os.path.join(LIB, "cpi")
-    ex_src = os.path.join(PKG, "examples", "cpi_main.cpp")
-    if force or _stale(exe, [ex_src, core] + _headers()):
-        subprocess.run([HIPCC] + COMMON + ["-x", "hip", ex_src, "-o", exe, "-L", LIB, "-lgpumpi",
-                                           "-Wl,-rpath,$ORIGIN"], check=True, capture_output=True)
+    # Converted C programs (converter.py + gpumpicc.py): standalone
+    # executables with the reference's `---gpumpi` command line.  lib/ring is
+    # this package's example; lib/cpi is the reference's own pi program,
+    # converted from its source when that is at hand.
+    import gpumpicc
+    examples = [(os.path.join(PKG, "examples", "ring.c"), os.path.join(LIB, "ring"))]
+    ref_cpi = os.path.join(REFERENCE, "toolchain_tests", "pi", "cpi.c")
+    if os.path.exists(ref_cpi):
+        examples.append((ref_cpi, os.path.join(LIB, "cpi")))
+    conv_deps = [core, os.path.join(PKG, "converter.py"), os.path.join(PKG, "gpumpicc.py")] + _headers()
+    for src, exe in examples:
+        if force or _stale(exe, [src] + conv_deps):
+            t0 = time.time()
+            gpumpicc.build([src], exe, workdir=OBJ, rpath="$ORIGIN")
+            if verbose:
+                print(f"[build] {os.path.basename(exe)} (converted {os.path.basename(src)}) {time.time() - t0:.1f}s",
+                      flush=True)
     return core, progs
 
 

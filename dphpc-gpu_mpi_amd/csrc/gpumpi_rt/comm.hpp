@@ -27,13 +27,16 @@ __device__ inline void* rank_malloc(uint64_t bytes) {
     const uint64_t cap = w->cfg.arena_bytes;
     uint64_t* top = &w->arena_top[my_rank()];
     const uint64_t need = (bytes + 255) & ~(uint64_t)255;
-    uint64_t off = *top;
+    // One lane's view of the top, one lane's store: the block is the same
+    // for every lane of the program's wave however its lanes are scheduled.
+    const uint64_t off = uniform_u64(*top);
     if (off + need + 16 > cap) {
         GPUMPI_FAIL(MPI_ERR_NO_MEM, "rank heap exhausted");
         return nullptr;
     }
     uint8_t* base = w->arena + (uint64_t)my_rank() * cap;
-    *top = off + need;  // every lane stores the same value (see device.hpp)
+    if (elected()) *top = off + need;
+    wave_sync();
     return base + off;
 }
 
@@ -42,7 +45,9 @@ __device__ inline void rank_free(void* p, uint64_t bytes) {
     uint8_t* base = w->arena + (uint64_t)my_rank() * w->cfg.arena_bytes;
     uint64_t* top = &w->arena_top[my_rank()];
     const uint64_t need = (bytes + 255) & ~(uint64_t)255;
-    if ((uint8_t*)p + need == base + *top) *top = (uint64_t)((uint8_t*)p - base);
+    const uint64_t t = uniform_u64(*top);
+    if ((uint8_t*)p + need == base + t && elected()) *top = (uint64_t)((uint8_t*)p - base);
+    wave_sync();
 }
 
 // ---------------------------------------------------------------------------

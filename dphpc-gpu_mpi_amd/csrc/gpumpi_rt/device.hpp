@@ -317,6 +317,15 @@ struct Spinner {
     }
 };
 
+// Scope guard of the point-to-point entry points: the program's wave leaves
+// the call together.  Inside, lanes take different paths (a lane alone
+// writes a flag), and on the GPU the wave reconverges by itself; the marker
+// costs no instruction there and keeps lane-by-lane execution (the CPU
+// simulation of the runtime) in step for the program's next statement.
+struct WaveJoin {
+    __device__ __forceinline__ ~WaveJoin() { wave_sync(); }
+};
+
 // MPI_Wtime clock: s_memrealtime, a constant-rate counter (wall_hz).
 __device__ __forceinline__ uint64_t wall_ticks() { return __builtin_readsteadycounter(); }
 

@@ -80,6 +80,7 @@ __device__ inline int irecv_common(void* buf, int count, MPI_Datatype dt, int so
 // Environment (mpi.cu:44-75, 142-146, 271-274)
 // ---------------------------------------------------------------------------
 __device__ inline int MPI_Init(int*, char***) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     ctx()->initialized = 1;
     return comm_barrier(world()->comm_world, ctx()->nranks) ? MPI_SUCCESS : MPI_ERR_OTHER;
@@ -93,6 +94,7 @@ __device__ inline int MPI_Initialized(int* flag) {
     return MPI_SUCCESS;
 }
 __device__ inline int MPI_Finalize(void) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     // Requests whose handles were freed still have to complete.
     Spinner sp;
@@ -164,6 +166,7 @@ __device__ inline int MPI_Op_free(MPI_Op* op) {
 // ---------------------------------------------------------------------------
 __device__ inline int MPI_Isend(const void* buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm,
                                 MPI_Request* request) {
+    gpumpi::rt::WaveJoin join_;
     int idx;
     int err = gpumpi::rt::isend_common(buf, count, dt, dest, tag, comm, false, &idx);
     if (err) return err;
@@ -172,6 +175,7 @@ __device__ inline int MPI_Isend(const void* buf, int count, MPI_Datatype dt, int
 }
 __device__ inline int MPI_Issend(const void* buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm,
                                  MPI_Request* request) {
+    gpumpi::rt::WaveJoin join_;
     int idx;
     int err = gpumpi::rt::isend_common(buf, count, dt, dest, tag, comm, true, &idx);
     if (err) return err;
@@ -180,6 +184,7 @@ __device__ inline int MPI_Issend(const void* buf, int count, MPI_Datatype dt, in
 }
 __device__ inline int MPI_Irecv(void* buf, int count, MPI_Datatype dt, int source, int tag, MPI_Comm comm,
                                 MPI_Request* request) {
+    gpumpi::rt::WaveJoin join_;
     int idx;
     int err = gpumpi::rt::irecv_common(buf, count, dt, source, tag, comm, &idx);
     if (err) return err;
@@ -187,6 +192,7 @@ __device__ inline int MPI_Irecv(void* buf, int count, MPI_Datatype dt, int sourc
     return MPI_SUCCESS;
 }
 __device__ inline int MPI_Send(const void* buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     uint64_t bytes;
     int err = msg_bytes(count, dt, &bytes);
@@ -201,6 +207,7 @@ __device__ inline int MPI_Send(const void* buf, int count, MPI_Datatype dt, int 
     return wait_req(idx, MPI_STATUS_IGNORE);
 }
 __device__ inline int MPI_Ssend(const void* buf, int count, MPI_Datatype dt, int dest, int tag, MPI_Comm comm) {
+    gpumpi::rt::WaveJoin join_;
     int idx;
     int err = gpumpi::rt::isend_common(buf, count, dt, dest, tag, comm, true, &idx);
     if (err) return err;
@@ -208,6 +215,7 @@ __device__ inline int MPI_Ssend(const void* buf, int count, MPI_Datatype dt, int
 }
 __device__ inline int MPI_Recv(void* buf, int count, MPI_Datatype dt, int source, int tag, MPI_Comm comm,
                                MPI_Status* status) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     uint64_t bytes;
     int err = msg_bytes(count, dt, &bytes);
@@ -226,6 +234,7 @@ __device__ inline int MPI_Recv(void* buf, int count, MPI_Datatype dt, int source
 __device__ inline int MPI_Sendrecv(const void* sendbuf, int sendcount, MPI_Datatype sendtype, int dest,
                                    int sendtag, void* recvbuf, int recvcount, MPI_Datatype recvtype, int source,
                                    int recvtag, MPI_Comm comm, MPI_Status* status) {
+    gpumpi::rt::WaveJoin join_;
     int ri, si;
     int err = gpumpi::rt::irecv_common(recvbuf, recvcount, recvtype, source, recvtag, comm, &ri);
     if (err) return err;
@@ -236,6 +245,7 @@ __device__ inline int MPI_Sendrecv(const void* sendbuf, int sendcount, MPI_Datat
     return e2 ? e2 : e1;
 }
 __device__ inline int MPI_Wait(MPI_Request* request, MPI_Status* status) {
+    gpumpi::rt::WaveJoin join_;
     if (*request == MPI_REQUEST_NULL) {
         if (status) *status = MPI_Status();
         return MPI_SUCCESS;
@@ -246,6 +256,7 @@ __device__ inline int MPI_Wait(MPI_Request* request, MPI_Status* status) {
     return gpumpi::rt::wait_req(idx, status);
 }
 __device__ inline int MPI_Test(MPI_Request* request, int* flag, MPI_Status* status) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     if (*request == MPI_REQUEST_NULL) {
         *flag = 1;
@@ -275,6 +286,7 @@ __device__ inline int MPI_Test(MPI_Request* request, int* flag, MPI_Status* stat
     return MPI_SUCCESS;
 }
 __device__ inline int MPI_Waitall(int count, MPI_Request reqv[], MPI_Status* statuses) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     Spinner sp;
     for (;;) {
@@ -308,6 +320,7 @@ __device__ inline int MPI_Waitall(int count, MPI_Request reqv[], MPI_Status* sta
     return err;
 }
 __device__ inline int MPI_Testall(int count, MPI_Request reqv[], int* flag, MPI_Status statuses[]) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     progress();
     for (int i = 0; i < count; i++) {
@@ -323,6 +336,7 @@ __device__ inline int MPI_Testall(int count, MPI_Request reqv[], int* flag, MPI_
     return MPI_Waitall(count, reqv, statuses);
 }
 __device__ inline int MPI_Waitany(int count, MPI_Request reqv[], int* index, MPI_Status* status) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     Spinner sp;
     for (;;) {
@@ -351,6 +365,7 @@ __device__ inline int MPI_Waitany(int count, MPI_Request reqv[], int* index, MPI
 }
 __device__ inline int MPI_Waitsome(int incount, MPI_Request reqv[], int* outcount, int indices[],
                                    MPI_Status statuses[]) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     Spinner sp;
     for (;;) {
@@ -385,6 +400,7 @@ __device__ inline int MPI_Waitsome(int incount, MPI_Request reqv[], int* outcoun
 // completes, and progress releases the request when it has (mpi.cu:608-617
 // freed the request object; here the pool entry is detached).
 __device__ inline int MPI_Request_free(MPI_Request* request) {
+    gpumpi::rt::WaveJoin join_;
     using namespace gpumpi::rt;
     if (*request == MPI_REQUEST_NULL) return MPI_SUCCESS;
     const int idx = req_index(*request);

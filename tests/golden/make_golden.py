@@ -8,6 +8,9 @@ the pi fixture, the reference's cpi.c under /root/reference):
   * cpi_mpich.json   -- the reference's toolchain_tests/pi/cpi.c compiled with
     host mpicc (oracle/Makefile `ref`) and run with N=1e8 at several rank
     counts; the printed pi is recorded.
+  * ring_mpich.json -- dphpc-gpu_mpi_amd/examples/ring.c (the source
+    converter's example C program) compiled with host mpicc and run with 5
+    rounds at 1, 4, 7 and 64 ranks: rank 0's output line.
   * semantics_mpich.json -- the point-to-point / request / communicator
     scenarios (dphpc-gpu_mpi_amd/csrc/progs/semantics_scenarios.h) under
     host MPI (oracle/hostmpi/semantics_golden.c) at 4, 8 and 13 ranks: every
@@ -46,11 +49,28 @@ def semantics():
         json.dump({"source": "semantics_scenarios.h under MPICH 3.3.2", "runs": out}, f)
 
 
+def ring():
+    src = os.path.join(ROOT, "dphpc-gpu_mpi_amd", "examples", "ring.c")
+    exe = os.path.join(ORACLE, "_build", "hostmpi_ring")
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    run(["/opt/conda/bin/mpicc", "-cc=gcc", "-O2", "-o", exe, src])
+    out = {}
+    for np_ in (1, 4, 7, 64):
+        txt = run([MPIRUN, "-np", str(np_), exe, "5"], capture_output=True, text=True).stdout
+        out[str(np_)] = [l for l in txt.splitlines() if l.startswith("ring:")][0]
+    with open(os.path.join(HERE, "ring_mpich.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "semantics":
         semantics()
         return 0
+    if len(sys.argv) > 1 and sys.argv[1] == "ring":
+        ring()
+        return 0
     semantics()
+    ring()
     run(["make", "-C", ORACLE, "_build/hostmpi_reduce_golden", "ref"])
     run([MPIRUN, "-np", "5", os.path.join(ORACLE, "_build", "hostmpi_reduce_golden"),
          os.path.join(HERE, "reduce_mpich.bin")])

@@ -1,0 +1,50 @@
+"""Converted C programs as standalone executables on the GPU: host main()
+of include/gpumpi/gpu_main.h (gpumpi_main) with the reference's `---gpumpi`
+command line (gpu_libs/gpu_main/main.cu:48-97), text through the host
+service channel.  lib/ring is dphpc-gpu_mpi_amd/examples/ring.c, lib/cpi the
+reference's toolchain_tests/pi/cpi.c, both built by converter.py and
+gpumpicc.py (dphpc-gpu_mpi_amd/build.py)."""
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "dphpc-gpu_mpi_amd", "lib")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _run(args):
+    print("[gpu] " + " ".join(os.path.basename(a) if i == 0 else a for i, a in enumerate(args)), flush=True)
+    return subprocess.run(args, capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("P", [4, 64])
+def test_converted_ring_matches_host_mpi(P):
+    exe = os.path.join(LIB, "ring")
+    if not os.path.exists(exe):
+        pytest.skip("lib/ring is not built")
+    p = _run([exe, "5", "---gpumpi", "-n", str(P), "-b", "64"])
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    gold = json.load(open(os.path.join(GOLDEN, "ring_mpich.json")))
+    assert gold[str(P)] in p.stdout, (gold[str(P)], p.stdout[-2000:])
+
+
+def test_converted_reference_cpi(orc):
+    exe = os.path.join(LIB, "cpi")
+    if not os.path.exists(exe):
+        pytest.skip("lib/cpi is not built (the reference's cpi.c was not at hand)")
+    P, n = 64, 100000000
+    p = _run([exe, str(n), "---gpumpi", "-n", str(P), "-b", "256"])
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    for r in range(P):
+        assert f"Process {r} of {P} is on" in p.stdout
+    pi = float(re.search(r"pi is approximately ([0-9.]+)", p.stdout).group(1))
+    want, _ = orc.pi_fold(n, P)
+    assert pi == float(f"{want:.16f}")
+    host = float(json.load(open(os.path.join(GOLDEN, "cpi_mpich.json")))["pi_printed"]["64"])
+    assert abs(pi - host) <= 1e-13 * abs(host)

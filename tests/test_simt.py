@@ -18,14 +18,26 @@ OUT = os.path.join(HERE, "hostbuild", "_build", "simt")
 ENV = dict(os.environ, ASAN_OPTIONS="detect_odr_violation=0:detect_leaks=0", SIMT_TIMEOUT="600")
 
 
+REF_CPI = "/root/reference/toolchain_tests/pi/cpi.c"
+
+
 @pytest.fixture(scope="module")
 def sim():
     # One build of every simulated program and driver: build.sh compiles all
-    # translation units in parallel.
+    # translation units in parallel.  The drivers include C programs run
+    # through the source converter (each has its own host main()).
+    import converter
+    os.makedirs(OUT, exist_ok=True)
+    drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp")]
+    ring = os.path.join(OUT, "conv_ring.cpp")
+    converter.convert(os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
+    drivers.append(ring)
+    if os.path.exists(REF_CPI):
+        cpi = os.path.join(OUT, "conv_cpi.cpp")
+        converter.convert(REF_CPI, cpi)
+        drivers.append(cpi)
     env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip")
-    subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] +
-                   [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp")],
-                   check=True, env=env, timeout=1200)
+    subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
 
 
@@ -184,3 +196,39 @@ def test_sim_io_scenarios(sim, tmp_path, P, N):
     for r in range(P):
         assert f"io_scenarios: rank {r} of {P}" in out
     assert os.listdir(tmp_path) == []  # every file the program made is deleted
+
+
+# The converted programs keep their data on the rank heap and in per-rank
+# globals, which every lane of the program's wave updates together; a heap
+# below the simulation's bulk-allocation size keeps those accesses in lane
+# lockstep, as the hardware runs them.
+SMALL_HEAP = ["-p", str(4 << 20)]
+
+
+@pytest.mark.parametrize("P", [1, 4, 7])
+def test_sim_converted_c_program_matches_host_mpi(sim, P):
+    # dphpc-gpu_mpi_amd/examples/ring.c through converter.py: per-rank
+    # globals, a static local, malloc/calloc/memset, a C++-keyword variable,
+    # MPI_Sendrecv around a ring and MPI_Allreduce; the output line must equal
+    # the same C program's under host MPICH (tests/golden/ring_mpich.json).
+    import json
+    gold = json.load(open(os.path.join(HERE, "golden", "ring_mpich.json")))
+    rc, out = _run([os.path.join(sim, "conv_ring"), "5", "---gpumpi", "-n", str(P), "-b", "64"] + SMALL_HEAP)
+    assert rc == 0, out[-3000:]
+    assert gold[str(P)] in out, (gold[str(P)], out[-2000:])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CPI), reason="the reference's cpi.c is not at hand")
+def test_sim_reference_cpi_converts_and_runs(sim, orc):
+    # The reference's own toolchain_tests/pi/cpi.c, unmodified, through the
+    # converter: every rank reports in, and pi is the rank-order FP64 fold of
+    # the cyclic partials (the oracle, which host MPICH's cpi pins).
+    import re
+    P, n = 4, 100000
+    rc, out = _run([os.path.join(sim, "conv_cpi"), str(n), "---gpumpi", "-n", str(P), "-b", "64"] + SMALL_HEAP)
+    assert rc == 0, out[-3000:]
+    for r in range(P):
+        assert f"Process {r} of {P} is on" in out
+    pi = float(re.search(r"pi is approximately ([0-9.]+)", out).group(1))
+    want, _ = orc.pi_fold(n, P)
+    assert pi == float(f"{want:.16f}"), (pi, want)
