@@ -21,9 +21,14 @@ each with its own CPU baseline:
   npb_is   NPB IS class B at 128 ranks: keys/s, Alltoallv GB/s and roofline,
            per-phase times; CPU baseline: the oracle's restatement of IS
            (one host thread);
-  allreduce  the exact FP32 Allreduce of 256 ranks x 16 MiB, and the
-           compressed-wire Allreduce (BF16, FP8 e4m3) against the exact
-           path at 256 ranks x 1M floats, with their error against FP64.
+  allreduce  the exact FP32 Allreduce of 256 ranks x 16 MiB; with --wire
+           also the compressed-wire Allreduce (BF16, FP8 e4m3) against the
+           exact path at 256 ranks x 1M floats, with their error against FP64;
+  pic      the particle-in-cell cycle at 8 x 8 ranks x 128 x 128 cells x
+           1M particles per rank, 100 cycles: ms per cycle, phase split,
+           communication share, HBM roofline on 48 B per particle per cycle,
+           energies against the oracle; CPU baseline: the oracle's cycle.
+Each side configuration runs in a child process of its own (--config NAME).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -226,11 +231,12 @@ def is_config(gm, np, threads, cpu):
             "cpu_baseline": cpu}
 
 
-def allreduce_config(gm, np, threads):
+def allreduce_config(gm, np, threads, wire=False):
     """Exact FP32 Allreduce of 256 ranks x 16 MiB (algorithmic 2*P*N*4
-    bytes), and at 256 ranks x 1M floats (N(0,1), seeded per rank) the exact
-    path against the BF16 and FP8-e4m3 wires: time per call, speed-up over
-    exact, and the normwise error of the result against the FP64 sum."""
+    bytes), and, with `wire`, at 256 ranks x 1M floats (N(0,1), seeded per
+    rank) the exact path against the BF16 and FP8-e4m3 wires: time per call,
+    speed-up over exact, and the normwise error of the result against the
+    FP64 sum."""
     out = {}
     P, n, iters = 256, 1 << 22, 10
     ib = gm.DeviceBuffer(4 * n * P)
@@ -245,6 +251,10 @@ def allreduce_config(gm, np, threads):
                                   "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS,
                                                "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4)}}
     del ib, ob
+    if not wire:
+        out["wire_256x1M_f32"] = {"status": "not run by default (--wire runs it): the compressed-wire Allreduce "
+                                            "shares the program of an unexplained GPU memory fault"}
+        return out
     n = 1 << 20
     ib = gm.DeviceBuffer(4 * n * P)
     ob = gm.DeviceBuffer(4 * n * P)
@@ -270,6 +280,137 @@ def allreduce_config(gm, np, threads):
     return out
 
 
+def pic_host_baseline():
+    """The oracle's restatement of the C5 cycle at its full shape (8 x 8
+    ranks, 128 x 128 cells and 1M particles per rank) on one host thread:
+    seconds per cycle from a 2-cycle run less the set-up (a 0-cycle run).
+    Also returns the two cycles' FP64 energies, which the device's first two
+    cycles must equal bit for bit."""
+    import time
+    for p in (os.path.join(ROOT, "oracle"),):
+        if p not in sys.path:
+            sys.path.insert(0, p)
+    import oracle
+    oracle.load()
+    t0 = time.perf_counter()
+    oracle.pic_run(oracle.pic_params(*PIC_SHAPE[:5], 0))
+    t1 = time.perf_counter()
+    d = oracle.pic_run(oracle.pic_params(*PIC_SHAPE[:5], 2))
+    t2 = time.perf_counter()
+    per_cycle = ((t2 - t1) - (t1 - t0)) / 2
+    parts = PIC_SHAPE[0] * PIC_SHAPE[1] * PIC_SHAPE[4]
+    return {"value": round(per_cycle * 1e3, 2), "unit": "ms per cycle", "cores": 1, "kind": "port",
+            "sample": f"the oracle's PIC cycle at the full shape ({parts} particles), 2 cycles less a 0-cycle "
+                      f"set-up run: {t2 - t1:.2f} s and {t1 - t0:.2f} s"}, [float(x) for x in d["energy"]]
+
+
+# C5: px, py, nx, ny, particles per rank, cycles
+PIC_SHAPE = (8, 8, 128, 128, 1000000, 100)
+PIC_PHASES = ["push and partition", "particle exchange (Sendrecv)", "deposit and energy fold",
+              "energy Allreduce", "density halo exchange (Sendrecv)", "field"]
+# Bytes each particle must cross HBM per cycle: the push reads and writes it
+# (2 x 16 B) and the deposition reads it again (16 B).
+PIC_BYTES_PER_PARTICLE = 48
+
+
+def pic_config(gm, np, threads, cpu):
+    """C5 particle-in-cell at 8 x 8 ranks x 128 x 128 cells x 1M particles per
+    rank, 100 cycles (`pic_exact`: FP64 energy Allreduce): time per cycle,
+    rank 0's phase split, the communication share, and the first two
+    cycles' energies against the oracle's."""
+    import struct
+    px, py, nx, ny, npr, cycles = PIC_SHAPE
+    P = px * py
+    cap = 2 * npr + 1024
+    e = gm.DeviceBuffer(8 * cycles)
+    cnt = gm.DeviceBuffer(8 * P)
+    tp = gm.DeviceBuffer(8 * 7)
+    for b in (e, cnt, tp):
+        b.zero()
+    f = lambda x: struct.unpack("<I", struct.pack("<f", x))[0]  # noqa: E731
+    res = gm.run("pic_exact", P, [0, px, py, nx, ny, npr, cycles, 2024, f(0.02), f(0.01), f(0.001), e, 0, cnt, 0, 0,
+                                  tp, cap], threads_per_rank=threads, heap_bytes=P * (49 * cap + (2 << 20)),
+                 timeout_s=300.0)
+    energy = e.download(np.float64)
+    t = tp.download(np.float64)
+    counts = cnt.download(np.int64)
+    cpu_row, cpu_energy = cpu if cpu else (None, None)
+    per_cycle = t[6] / cycles
+    comm = t[1] + t[3] + t[4]
+    gbps = P * npr * PIC_BYTES_PER_PARTICLE / per_cycle / 1e9
+    out = {"ranks": P, "cells_per_rank": nx * ny, "particles_per_rank": npr, "cycles": cycles,
+           "threads_per_rank": threads, "status": res.status,
+           "particles_conserved": bool(int(counts.sum()) == P * npr),
+           "ms_per_cycle": round(per_cycle * 1e3, 4),
+           "phase_ms_per_cycle_rank0": {k: round(float(v) / cycles * 1e3, 4) for k, v in zip(PIC_PHASES, t[:6])},
+           "communication_share": round(float(comm / t[6]), 4),
+           "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None,
+                        "algorithmic_bytes": f"{PIC_BYTES_PER_PARTICLE} B per particle per cycle"},
+           "energy_first": float(energy[0]), "energy_last": float(energy[-1]),
+           "cpu_baseline": cpu_row}
+    if cpu_energy is not None:
+        out["energy_bit_equal_to_oracle_first_2_cycles"] = bool(
+            np.array_equal(energy[:2].view(np.uint64), np.asarray(cpu_energy, dtype=np.float64).view(np.uint64)))
+    return out
+
+
+CONFIGS = ("pi", "p2p", "npb_is", "allreduce", "pic")
+
+
+def run_config(name, args):
+    """One side configuration in this process: its host baseline first,
+    then the GPU.  Returns the configuration's JSON object."""
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = {"pi": cpi_host_baseline, "npb_is": is_host_baseline, "pic": pic_host_baseline}.get(name,
+                                                                                                   lambda: None)()
+    import numpy as np
+    import gpumpi as gm
+    gm.load()
+    try:
+        if name == "pi":
+            return pi_config(gm, np, cpu)
+        if name == "p2p":
+            P, S = 256, 64 << 20
+            sb = gm.DeviceBuffer(S * P)
+            rb = gm.DeviceBuffer(S * P)
+            gm.fill_payload(sb, S, S, SEED, 0, P)
+            return p2p_config(gm, np, sb, rb, S, P)
+        if name == "npb_is":
+            return is_config(gm, np, args.threads, cpu)
+        if name == "allreduce":
+            return allreduce_config(gm, np, args.threads, args.wire)
+        if name == "pic":
+            return pic_config(gm, np, args.threads, cpu)
+        raise ValueError(name)
+    finally:
+        gm.release_world()
+
+
+def side_configs(args):
+    """Every side configuration in a child process of its own (this script
+    with --config NAME), so that a failure in one is reported in its entry
+    and leaves the others and the headline line intact."""
+    out = {}
+    for name in CONFIGS:
+        print(f"[bench] config {name}", file=sys.stderr, flush=True)
+        cmd = [sys.executable, os.path.abspath(__file__), "--config", name, "--threads", str(args.threads)]
+        cmd += ["--no-cpu-baseline"] if args.no_cpu_baseline else []
+        cmd += ["--wire"] if args.wire else []
+        try:
+            p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=900)
+        except subprocess.TimeoutExpired:
+            out[name] = {"status": "failed: no result within 900 s"}
+            continue
+        lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        if p.returncode == 0 and lines:
+            out[name] = json.loads(lines[-1])
+        else:
+            out[name] = {"status": f"failed: exit status {p.returncode}", "output_tail": p.stdout[-400:]}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +424,9 @@ def main(argv=None):
     ap.add_argument("--no-extras", action="store_true", help="skip the Allreduce / NPB IS side measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (reference for HBM counters) and exit")
+    ap.add_argument("--wire", action="store_true",
+                    help="also time the BF16 / FP8 compressed-wire Allreduce against the exact path")
+    ap.add_argument("--config", choices=CONFIGS, help="run one side configuration alone and print its JSON")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the whole MPI world lives in one device); "
@@ -290,13 +434,12 @@ def main(argv=None):
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.config:
+        print(json.dumps(run_config(args.config, args)), flush=True)
+        return 0
 
-    # Host baselines run before the GPU is touched.
+    # The host baseline runs before the GPU is touched.
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.bytes)
-    cpu_pi = cpu_is = None
-    if not args.no_cpu_baseline and not args.no_extras:
-        cpu_pi = cpi_host_baseline()
-        cpu_is = is_host_baseline()
 
     import numpy as np
     import gpumpi as gm
@@ -367,19 +510,14 @@ def main(argv=None):
         "ring_gbps_mpi_wtime": round(wtime_gbps, 2),
         "payload_mismatches": bad,
     }
-    if not args.no_extras:
-        line["configs"] = {
-            "pi": pi_config(gm, np, cpu_pi),
-            "p2p": p2p_config(gm, np, sb, rb, stride, P),
-            "npb_is": is_config(gm, np, args.threads, cpu_is),
-            "allreduce": allreduce_config(gm, np, args.threads),
-            "batched_allreduce": {"status": "not measured: the batched uneven Allreduce at its 1024-rank shape "
-                                            "is kept off the GPU (unexplained GPU memory faults of that family); "
-                                            "its parity runs at 96 ranks in the GPU tests"},
-        }
-    print(json.dumps(line), flush=True)
     del sb, rb, ppb
     gm.release_world()
+    if not args.no_extras:
+        line["configs"] = side_configs(args)
+        line["configs"]["batched_allreduce"] = {
+            "status": "not measured: the batched uneven Allreduce at its 1024-rank shape is kept off the GPU "
+                      "(unexplained GPU memory faults of that family); its parity runs at 96 ranks in the GPU tests"}
+    print(json.dumps(line), flush=True)
     return 0 if bad == 0 else 1
 
 

@@ -659,3 +659,264 @@ void orc_comm_split(int nranks, const int* colors, const int* keys,
         new_size[r] = size;
     }
 }
+
+/* ------------------------------------------------------------------------
+ * 2-D particle-in-cell cycle (the C5 workload), decomposed over a px x py
+ * grid of ranks exactly as the rank program csrc/progs/pic.hip runs it, and
+ * the same physics on the undecomposed global grid.  The workload is this
+ * build's own (the reference's sputniPIC, toolchain_tests/sputnipic/src/
+ * mpi_comm.c:21-307, decomposes particles, not space); the oracle.h block
+ * states every rule both sides follow.
+ * ---------------------------------------------------------------------- */
+static float pic_unit(uint64_t w) { return (float)(w >> 40) * (1.0f / 16777216.0f); }
+static uint64_t pic_draw(uint64_t seed, int rank, int64_t i, int k) {
+    return orc_mix64(seed + (uint64_t)rank * 0xD1B54A32D192ED03ULL +
+                     (uint64_t)(i * 32 + k + 1) * 0x9E3779B97F4A7C15ULL);
+}
+void orc_pic_init_particle(const orc_pic_params* p, int rank, int64_t i, float* xyuv) {
+    const int rx = rank % p->px, ry = rank / p->px;
+    const float x0 = (float)(rx * p->nx), y0 = (float)(ry * p->ny);
+    float x = x0 + (float)p->nx * pic_unit(pic_draw(p->seed, rank, i, 0));
+    float y = y0 + (float)p->ny * pic_unit(pic_draw(p->seed, rank, i, 1));
+    if (x >= x0 + (float)p->nx) x = x0;
+    if (y >= y0 + (float)p->ny) y = y0;
+    float gu = 0.0f, gv = 0.0f;
+    for (int k = 0; k < 12; k++) gu = gu + pic_unit(pic_draw(p->seed, rank, i, 2 + k));
+    for (int k = 0; k < 12; k++) gv = gv + pic_unit(pic_draw(p->seed, rank, i, 14 + k));
+    xyuv[0] = x;
+    xyuv[1] = y;
+    xyuv[2] = p->drift + p->vth * (gu - 6.0f);
+    xyuv[3] = p->vth * (gv - 6.0f);
+}
+
+/* direction index of (dx, dy) in {-1,0,1}^2 \ (0,0): 1..8, row by row */
+static const int PIC_DX[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
+static const int PIC_DY[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+
+static void pic_push(const orc_pic_params* p, float* q, const float* ex, const float* ey, int w, int x0, int y0) {
+    const float Lx = (float)(p->px * p->nx), Ly = (float)(p->py * p->ny);
+    const int lx = (int)q[0] - x0, ly = (int)q[1] - y0;
+    const float e1 = ex[(size_t)ly * w + lx], e2 = ey[(size_t)ly * w + lx];
+    float u = q[2] + p->qm * e1;
+    float v = q[3] + p->qm * e2;
+    float x = q[0] + u, y = q[1] + v;
+    if (x < 0.0f) x = x + Lx;
+    if (x >= Lx) x = x - Lx;
+    if (y < 0.0f) y = y + Ly;
+    if (y >= Ly) y = y - Ly;
+    q[0] = x;
+    q[1] = y;
+    q[2] = u;
+    q[3] = v;
+}
+
+/* phi over the interior plus one ring, then E = -grad(phi) on the interior,
+ * from rho with a two-cell ring; arrays (h+4) x (w+4), interior at [2, 2). */
+static void pic_field(const float* rho, float* phi, float* ex, float* ey, int w, int h) {
+    const int W = w + 4;
+    for (int j = 1; j < h + 3; j++)
+        for (int i = 1; i < w + 3; i++) {
+            const float* r = rho + (size_t)j * W + i;
+            phi[(size_t)j * W + i] = ((((4.0f * r[0] + r[-1]) + r[1]) + r[-W]) + r[W]) * 0.125f;
+        }
+    for (int j = 0; j < h; j++)
+        for (int i = 0; i < w; i++) {
+            const float* f = phi + (size_t)(j + 2) * W + (i + 2);
+            ex[(size_t)j * w + i] = (f[-1] - f[1]) * 0.5f;
+            ey[(size_t)j * w + i] = (f[-W] - f[W]) * 0.5f;
+        }
+}
+
+static uint64_t pic_hash(const void* data, size_t n) {
+    const uint8_t* b = (const uint8_t*)data;
+    uint64_t h = 1469598103934665603ULL;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ULL;
+    return h;
+}
+
+/* energy of n particles: 1024-particle blocks summed in order */
+double orc_pic_energy(const float* q, int64_t n) {
+    double total = 0.0;
+    for (int64_t b = 0; b < n; b += 1024) {
+        double s = 0.0;
+        for (int64_t i = b; i < n && i < b + 1024; i++) {
+            const double u = q[4 * i + 2], v = q[4 * i + 3];
+            s = s + 0.5 * (u * u + v * v);
+        }
+        total = b == 0 ? s : total + s;
+    }
+    return total;
+}
+
+typedef struct {
+    float* q;       /* particles, 4 floats each */
+    int64_t n, cap;
+    float* rho;     /* (ny+4) x (nx+4) */
+    float* phi;
+    float* ex;      /* ny x nx */
+    float* ey;
+    int* count;     /* ny x nx */
+} pic_rank;
+
+static void pic_deposit(const orc_pic_params* p, pic_rank* R, int x0, int y0) {
+    const int w = p->nx, h = p->ny, W = w + 4;
+    memset(R->count, 0, sizeof(int) * (size_t)w * h);
+    for (int64_t i = 0; i < R->n; i++) {
+        const int lx = (int)R->q[4 * i] - x0, ly = (int)R->q[4 * i + 1] - y0;
+        R->count[(size_t)ly * w + lx]++;
+    }
+    memset(R->rho, 0, sizeof(float) * (size_t)(h + 4) * W);
+    for (int j = 0; j < h; j++)
+        for (int i = 0; i < w; i++) R->rho[(size_t)(j + 2) * W + i + 2] = (float)R->count[(size_t)j * w + i];
+}
+
+/* The strip of rank A's interior that its neighbour in direction d needs,
+ * copied into neighbour B's ring on the side facing A (B = A + d). */
+static void pic_halo(const orc_pic_params* p, pic_rank* R) {
+    const int P = p->px * p->py, w = p->nx, h = p->ny, W = w + 4;
+    for (int d = 1; d <= 8; d++) {
+        for (int b = 0; b < P; b++) {
+            const int bx = b % p->px, by = b / p->px;
+            /* receives, in step d, from the neighbour opposite d */
+            const int ax = (bx - PIC_DX[d] + p->px) % p->px, ay = (by - PIC_DY[d] + p->py) % p->py;
+            const pic_rank* A = &R[ay * p->px + ax];
+            const int dx = PIC_DX[d], dy = PIC_DY[d];
+            /* source rows/cols in A's interior (facing +d), target in B's ring (facing -d) */
+            const int sx0 = dx < 0 ? 2 : dx > 0 ? w : 2, nxs = dx == 0 ? w : 2;
+            const int sy0 = dy < 0 ? 2 : dy > 0 ? h : 2, nys = dy == 0 ? h : 2;
+            const int tx0 = dx < 0 ? w + 2 : dx > 0 ? 0 : 2;
+            const int ty0 = dy < 0 ? h + 2 : dy > 0 ? 0 : 2;
+            for (int j = 0; j < nys; j++)
+                for (int i = 0; i < nxs; i++)
+                    R[b].rho[(size_t)(ty0 + j) * W + tx0 + i] = A->rho[(size_t)(sy0 + j) * W + sx0 + i];
+        }
+    }
+}
+
+int orc_pic_run(const orc_pic_params* p, double* energy, float* energy_bf16, int64_t* counts,
+                uint64_t* part_hash, uint64_t* rho_hash, float* final_q_out) {
+    const int P = p->px * p->py, w = p->nx, h = p->ny, W = w + 4;
+    pic_rank* R = (pic_rank*)calloc((size_t)P, sizeof(pic_rank));
+    if (!R) return -1;
+    int64_t cap = p->np * 2 + 1024;
+    for (int r = 0; r < P; r++) {
+        R[r].cap = cap;
+        R[r].q = (float*)malloc(sizeof(float) * 4 * (size_t)cap);
+        R[r].rho = (float*)calloc((size_t)(h + 4) * W, sizeof(float));
+        R[r].phi = (float*)calloc((size_t)(h + 4) * W, sizeof(float));
+        R[r].ex = (float*)calloc((size_t)h * w, sizeof(float));
+        R[r].ey = (float*)calloc((size_t)h * w, sizeof(float));
+        R[r].count = (int*)calloc((size_t)h * w, sizeof(int));
+        if (!R[r].q || !R[r].rho || !R[r].phi || !R[r].ex || !R[r].ey || !R[r].count) return -1;
+        R[r].n = p->np;
+        for (int64_t i = 0; i < p->np; i++) orc_pic_init_particle(p, r, i, R[r].q + 4 * i);
+    }
+    for (int r = 0; r < P; r++) pic_deposit(p, &R[r], (r % p->px) * w, (r / p->px) * h);
+    pic_halo(p, R);
+    for (int r = 0; r < P; r++) pic_field(R[r].rho, R[r].phi, R[r].ex, R[r].ey, w, h);
+    float** out = (float**)calloc((size_t)P * 9, sizeof(float*));
+    int64_t* nout = (int64_t*)calloc((size_t)P * 9, sizeof(int64_t));
+    for (int c = 0; c < p->cycles; c++) {
+        /* push, classify, stable partition */
+        for (int r = 0; r < P; r++) {
+            const int rx = r % p->px, ry = r / p->px, x0 = rx * w, y0 = ry * h;
+            for (int d = 0; d < 9; d++) {
+                free(out[r * 9 + d]);
+                out[r * 9 + d] = (float*)malloc(sizeof(float) * 4 * (size_t)(R[r].n + 1));
+                nout[r * 9 + d] = 0;
+            }
+            for (int64_t i = 0; i < R[r].n; i++) {
+                float* q = R[r].q + 4 * i;
+                pic_push(p, q, R[r].ex, R[r].ey, w, x0, y0);
+                const int ox = (int)q[0] / w, oy = (int)q[1] / h;
+                const int dx = ox == rx ? 0 : ox == (rx + 1) % p->px ? 1 : -1;
+                const int dy = oy == ry ? 0 : oy == (ry + 1) % p->py ? 1 : -1;
+                int d = 0;
+                for (int k = 1; k <= 8; k++)
+                    if (PIC_DX[k] == dx && PIC_DY[k] == dy) d = k;
+                memcpy(out[r * 9 + d] + 4 * nout[r * 9 + d]++, q, sizeof(float) * 4);
+            }
+        }
+        /* exchange: stays first, then step d's arrivals from the neighbour opposite d */
+        for (int r = 0; r < P; r++) {
+            const int rx = r % p->px, ry = r / p->px;
+            int64_t n = 0;
+            memcpy(R[r].q, out[r * 9], sizeof(float) * 4 * (size_t)nout[r * 9]);
+            n = nout[r * 9];
+            for (int d = 1; d <= 8; d++) {
+                const int ax = (rx - PIC_DX[d] + p->px) % p->px, ay = (ry - PIC_DY[d] + p->py) % p->py;
+                const int a = ay * p->px + ax;
+                if (n + nout[a * 9 + d] > R[r].cap) return -2;
+                memcpy(R[r].q + 4 * n, out[a * 9 + d], sizeof(float) * 4 * (size_t)nout[a * 9 + d]);
+                n += nout[a * 9 + d];
+            }
+            R[r].n = n;
+        }
+        /* deposit, energy, halo, field */
+        double e = 0.0;
+        float e16 = 0.0f;
+        for (int r = 0; r < P; r++) {
+            pic_deposit(p, &R[r], (r % p->px) * w, (r / p->px) * h);
+            const double er = orc_pic_energy(R[r].q, R[r].n);
+            e = r == 0 ? er : er + e;
+            const float d16 = orc_bf16_to_f32(orc_f32_to_bf16((float)er));
+            e16 = r == 0 ? d16 : d16 + e16;
+        }
+        if (energy) energy[c] = e;
+        if (energy_bf16) energy_bf16[c] = e16;
+        pic_halo(p, R);
+        for (int r = 0; r < P; r++) pic_field(R[r].rho, R[r].phi, R[r].ex, R[r].ey, w, h);
+    }
+    int64_t at = 0;
+    for (int r = 0; r < P; r++) {
+        if (final_q_out) memcpy(final_q_out + 4 * at, R[r].q, sizeof(float) * 4 * (size_t)R[r].n);
+        at += R[r].n;
+        if (counts) counts[r] = R[r].n;
+        if (part_hash) part_hash[r] = pic_hash(R[r].q, sizeof(float) * 4 * (size_t)R[r].n);
+        if (rho_hash) rho_hash[r] = pic_hash(R[r].rho, sizeof(float) * (size_t)(h + 4) * W);
+        free(R[r].q); free(R[r].rho); free(R[r].phi); free(R[r].ex); free(R[r].ey); free(R[r].count);
+    }
+    for (int k = 0; k < P * 9; k++) free(out[k]);
+    free(out); free(nout); free(R);
+    return 0;
+}
+
+/* The same physics on the undecomposed (px*nx) x (py*ny) periodic grid: one
+ * particle array in initial (rank, index) order, one density and field. */
+int orc_pic_global(const orc_pic_params* p, double* energy, float* final_q_sorted_out, int64_t* n_out) {
+    const int P = p->px * p->py, GW = p->px * p->nx, GH = p->py * p->ny, W = GW + 4;
+    const int64_t n = (int64_t)P * p->np;
+    float* q = (float*)malloc(sizeof(float) * 4 * (size_t)n);
+    float* rho = (float*)calloc((size_t)(GH + 4) * W, sizeof(float));
+    float* phi = (float*)calloc((size_t)(GH + 4) * W, sizeof(float));
+    float* ex = (float*)calloc((size_t)GH * GW, sizeof(float));
+    float* ey = (float*)calloc((size_t)GH * GW, sizeof(float));
+    int* cnt = (int*)calloc((size_t)GH * GW, sizeof(int));
+    if (!q || !rho || !phi || !ex || !ey || !cnt) return -1;
+    for (int r = 0; r < P; r++)
+        for (int64_t i = 0; i < p->np; i++) orc_pic_init_particle(p, r, i, q + 4 * ((int64_t)r * p->np + i));
+    for (int c = -1; c < p->cycles; c++) {
+        if (c >= 0)
+            for (int64_t i = 0; i < n; i++) pic_push(p, q + 4 * i, ex, ey, GW, 0, 0);
+        memset(cnt, 0, sizeof(int) * (size_t)GW * GH);
+        for (int64_t i = 0; i < n; i++) cnt[(size_t)(int)q[4 * i + 1] * GW + (int)q[4 * i]]++;
+        for (int j = 0; j < GH + 4; j++)
+            for (int i = 0; i < GW + 4; i++) {
+                const int gx = (i - 2 + GW) % GW, gy = (j - 2 + GH) % GH;
+                rho[(size_t)j * W + i] = (float)cnt[(size_t)gy * GW + gx];
+            }
+        pic_field(rho, phi, ex, ey, GW, GH);
+        if (c >= 0 && energy) {
+            double s = 0.0;
+            for (int64_t i = 0; i < n; i++) {
+                const double u = q[4 * i + 2], v = q[4 * i + 3];
+                s = s + 0.5 * (u * u + v * v);
+            }
+            energy[c] = s;
+        }
+    }
+    if (final_q_sorted_out) memcpy(final_q_sorted_out, q, sizeof(float) * 4 * (size_t)n);
+    if (n_out) *n_out = n;
+    free(q); free(rho); free(phi); free(ex); free(ey); free(cnt);
+    return 0;
+}

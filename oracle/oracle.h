@@ -102,6 +102,54 @@ void orc_e4m3_block_decode(const uint8_t* q, const float* scales, size_t n, int 
 void orc_comm_split(int nranks, const int* colors, const int* keys,
                     int* new_rank, int* new_size);
 
+/* ---- 2-D particle-in-cell cycle (the C5 workload) ---------------------
+ * This build's own workload (the reference has no spatially decomposed PIC;
+ * its sputniPIC, toolchain_tests/sputnipic/src/mpi_comm.c:21-307, scatters
+ * particles).  Ranks form a px x py grid; rank r = ry*px + rx owns cells
+ * [rx*nx, rx*nx+nx) x [ry*ny, ry*ny+ny) of a periodic domain, unit cells.
+ * Particles are (x, y, u, v) in FP32, positions global.  All FP arithmetic
+ * is written out in the order below and built without contraction.
+ *   init   draw k of particle i of rank r: mix64(seed + r*0xD1B54A32D192ED03
+ *          + (32i+k+1)*0x9E3779B97F4A7C15), unit = (w>>40)*2^-24;
+ *          x = x0 + nx*unit(k=0) (x0 if that rounds to x0+nx), y likewise
+ *          (k=1); u = drift + vth*(sum_k=2..13 unit - 6), v = vth*(sum_k=14..25
+ *          unit - 6) (Irwin-Hall normals, summed in k order).
+ *   cycle  push: u += qm*Ex(cell), v += qm*Ey(cell), x += u, y += v, wrapped
+ *          once into [0, L); classify by the owner of (int)x, (int)y: stay
+ *          or direction d = 1..8 of (dx, dy) in row order (-1,-1) .. (1,1);
+ *          stable partition; exchange: the rank keeps its stays in order,
+ *          then for d = 1..8 appends what the neighbour opposite d sent in
+ *          direction d; deposit NGP counts; energy = sum over particles of
+ *          0.5*(u*u + v*v) in FP64, 1024-particle blocks summed in order per
+ *          rank, then folded in rank order (MPI_Allreduce), and the same per
+ *          rank in FP32 through the BF16 wire (fold of dequantised values);
+ *          rho = counts with a two-cell ring filled, for d = 1..8, from the
+ *          neighbour opposite d; phi = ((((4c + w) + e) + s) + n)/8 on the
+ *          interior and one ring; E = ((phi[-1] - phi[+1])/2, (phi[-W] -
+ *          phi[+W])/2) on the interior.  The fields are also built once
+ *          before the first cycle.
+ * Pinned by: orc_pic_global, the same physics on the undecomposed grid
+ * (identical particle states as a multiset, identical densities, energies
+ * within FP64 summation-order rounding); particle-count conservation; and
+ * with qm = 0 the ballistic closed form (constant energy, per-particle
+ * velocities unchanged). */
+typedef struct orc_pic_params {
+    int px, py, nx, ny;
+    int64_t np;        /* initial particles per rank */
+    int cycles;
+    uint64_t seed;
+    float vth, drift, qm;
+} orc_pic_params;
+void orc_pic_init_particle(const orc_pic_params* p, int rank, int64_t i, float* xyuv);
+double orc_pic_energy(const float* q, int64_t n);
+/* energy[cycles] (FP64 fold), energy_bf16[cycles], and per rank the final
+ * particle count, an FNV-1a hash of its particle array and of its rho
+ * (with ring); final_q_out (4 floats per particle, ranks in order) if
+ * non-NULL.  Returns 0, or <0 on allocation failure. */
+int orc_pic_run(const orc_pic_params* p, double* energy, float* energy_bf16, int64_t* counts,
+                uint64_t* part_hash, uint64_t* rho_hash, float* final_q_out);
+int orc_pic_global(const orc_pic_params* p, double* energy, float* final_q_out, int64_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,11 @@ def load():
     lib.orc_e4m3_block_encode.argtypes = [vp, sz, ctypes.c_int, vp, vp]
     lib.orc_e4m3_block_decode.argtypes = [vp, vp, sz, ctypes.c_int, vp]
     lib.orc_comm_split.argtypes = [ctypes.c_int, vp, vp, vp, vp]
+    lib.orc_pic_run.argtypes = [ctypes.c_void_p, vp, vp, vp, vp, vp, vp]
+    lib.orc_pic_global.argtypes = [ctypes.c_void_p, vp, vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.orc_pic_energy.argtypes = [vp, ctypes.c_int64]
+    lib.orc_pic_energy.restype = ctypes.c_double
+    lib.orc_pic_init_particle.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, vp]
     _lib = lib
     return lib
 
@@ -191,3 +196,42 @@ def comm_split(colors, keys):
     ns = np.empty(n, dtype=np.int32)
     load().orc_comm_split(n, _p(colors), _p(keys), _p(nr), _p(ns))
     return nr, ns
+
+
+class PicParams(ctypes.Structure):
+    _fields_ = [("px", ctypes.c_int), ("py", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("np", ctypes.c_int64), ("cycles", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("vth", ctypes.c_float), ("drift", ctypes.c_float), ("qm", ctypes.c_float)]
+
+
+def pic_params(px, py, nx, ny, np_, cycles, seed=2024, vth=0.02, drift=0.01, qm=0.001):
+    return PicParams(px, py, nx, ny, np_, cycles, seed, vth, drift, qm)
+
+
+def pic_run(p, want_particles=False):
+    """The decomposed PIC run: dict of energy, energy_bf16, counts,
+    part_hash, rho_hash (per rank) and, optionally, the final particles."""
+    lib = load()
+    P = p.px * p.py
+    e = np.zeros(p.cycles, dtype=np.float64)
+    e16 = np.zeros(p.cycles, dtype=np.float32)
+    counts = np.zeros(P, dtype=np.int64)
+    ph = np.zeros(P, dtype=np.uint64)
+    rh = np.zeros(P, dtype=np.uint64)
+    q = np.zeros((P * p.np, 4), dtype=np.float32) if want_particles else None
+    rc = lib.orc_pic_run(ctypes.byref(p), _p(e), _p(e16), _p(counts), _p(ph), _p(rh),
+                         _p(q) if q is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"orc_pic_run failed ({rc})")
+    return {"energy": e, "energy_bf16": e16, "counts": counts, "part_hash": ph, "rho_hash": rh, "particles": q}
+
+
+def pic_global(p):
+    lib = load()
+    n = p.px * p.py * p.np
+    e = np.zeros(p.cycles, dtype=np.float64)
+    q = np.zeros((n, 4), dtype=np.float32)
+    nn = ctypes.c_int64(0)
+    if lib.orc_pic_global(ctypes.byref(p), _p(e), _p(q), ctypes.byref(nn)) != 0:
+        raise RuntimeError("orc_pic_global failed")
+    return e, q

@@ -4,6 +4,7 @@ Each check is independent of the oracle's own code: golden vectors produced
 by host MPI (MPICH) from the reference's programs or ours, the reference's
 known-answer arrays, closed forms, and library routines.
 """
+import ctypes
 import json
 import math
 import os
@@ -274,3 +275,46 @@ def test_e4m3_block_roundtrip_error(orc):
     rel = np.linalg.norm(y.astype(np.float64) - x) / np.linalg.norm(x.astype(np.float64))
     assert 1e-3 < rel < 0.05
     assert s.size == 64
+
+
+# ---- the particle-in-cell workload (oracle.h orc_pic_run) -----------------
+def _sorted_rows(q):
+    v = np.ascontiguousarray(q).view(np.uint32).reshape(-1, 4)
+    return v[np.lexsort(v.T[::-1])]
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 16, 16, 2000, 20), (3, 2, 8, 12, 500, 30), (1, 1, 16, 16, 300, 10),
+                                   (2, 1, 12, 8, 400, 25), (4, 4, 32, 32, 5000, 10)])
+def test_pic_decomposition_equals_the_global_grid(orc, shape):
+    # The decomposed run (per-rank grids, stable partition, 8-direction
+    # Sendrecv exchange, two-cell halos) must move every particle exactly as
+    # the same physics on one undecomposed periodic grid: the same particle
+    # states bit for bit (as a multiset), the same count, and energies that
+    # differ only by FP64 summation order.
+    px, py, nx, ny, np_, cycles = shape
+    p = orc.pic_params(px, py, nx, ny, np_, cycles)
+    d = orc.pic_run(p, want_particles=True)
+    eg, qg = orc.pic_global(p)
+    assert d["counts"].sum() == px * py * np_
+    assert np.array_equal(_sorted_rows(d["particles"]), _sorted_rows(qg))
+    assert np.max(np.abs(d["energy"] - eg) / eg) < 1e-13
+    # BF16 wire: within the codec's relative error of the exact sum
+    assert np.max(np.abs(d["energy_bf16"] - d["energy"]) / d["energy"]) < 1e-2
+
+
+def test_pic_ballistic_closed_form(orc):
+    # qm = 0: no force, so every velocity keeps its initial value (the
+    # energy is that of the initial draws) and every position advances by
+    # its velocity, wrapped.
+    p = orc.pic_params(2, 2, 16, 16, 1000, 15, qm=0.0)
+    d = orc.pic_run(p, want_particles=True)
+    init = np.zeros((4 * 1000, 4), dtype=np.float32)
+    buf = np.zeros(4, dtype=np.float32)
+    for r in range(4):
+        for i in range(1000):
+            orc.load().orc_pic_init_particle(ctypes.byref(p), r, i, buf.ctypes.data_as(ctypes.c_void_p))
+            init[r * 1000 + i] = buf
+    e0 = sum(0.5 * (float(u) * float(u) + float(v) * float(v)) for u, v in init[:, 2:])
+    assert np.allclose(d["energy"], e0, rtol=1e-13, atol=0)
+    vel = lambda q: np.sort(q[:, 2:].view(np.uint64).ravel())
+    assert np.array_equal(vel(d["particles"]), vel(init))

@@ -28,7 +28,8 @@ def sim():
     # through the source converter (each has its own host main()).
     import converter
     os.makedirs(OUT, exist_ok=True)
-    drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp")]
+    drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp",
+                                              "run_pic.cpp")]
     ring = os.path.join(OUT, "conv_ring.cpp")
     converter.convert(os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
     drivers.append(ring)
@@ -36,7 +37,7 @@ def sim():
         cpi = os.path.join(OUT, "conv_cpi.cpp")
         converter.convert(REF_CPI, cpi)
         drivers.append(cpi)
-    env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip")
+    env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip")
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
 
@@ -232,3 +233,56 @@ def test_sim_reference_cpi_converts_and_runs(sim, orc):
     pi = float(re.search(r"pi is approximately ([0-9.]+)", out).group(1))
     want, _ = orc.pi_fold(n, P)
     assert pi == float(f"{want:.16f}"), (pi, want)
+
+
+def _pic_out(out):
+    e, e16, ranks = {}, {}, {}
+    for line in out.splitlines():
+        f = line.split()
+        if f[:1] == ["energy"]:
+            e[int(f[1])] = int(f[2])
+        elif f[:1] == ["energy16"]:
+            e16[int(f[1])] = int(f[2])
+        elif f[:1] == ["rank"] and "phash" in f:
+            ranks[int(f[1])] = (int(f[3]), int(f[5]), int(f[7]))
+    return e, e16, ranks
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 16, 16, 2000, 5), (3, 2, 8, 12, 500, 12), (2, 1, 12, 8, 400, 8),
+                                   (1, 1, 10, 10, 300, 4)])
+def test_sim_pic_matches_oracle(sim, orc, shape):
+    # The C5 rank program (progs/pic.hip) against orc_pic_run, bit for bit:
+    # every cycle's FP64 energy and BF16-wire energy, and per rank the final
+    # particle count, particle array and density (with its halo ring).
+    _check_pic(sim, orc, shape, "pic")
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 16, 16, 2000, 5), (3, 3, 8, 8, 700, 6)])
+def test_sim_pic_exact_matches_oracle(sim, orc, shape):
+    # `pic_exact` (the same cycle without the BF16-wire Allreduce) leaves the
+    # BF16 energies unwritten and matches the oracle in everything else.
+    _check_pic(sim, orc, shape, "pic_exact")
+
+
+def _check_pic(sim, orc, shape, prog):
+    import struct
+    px, py, nx, ny, np_, cycles = shape
+    rc, out = _run([os.path.join(sim, "run_pic"), str(px), str(py), str(nx), str(ny), str(np_), str(cycles),
+                    "2024", "128", prog])
+    assert rc == 0, out[-3000:]
+    e, e16, ranks = _pic_out(out)
+    d = orc.pic_run(orc.pic_params(px, py, nx, ny, np_, cycles, seed=2024))
+    for c in range(cycles):
+        assert e[c] == struct.unpack("<Q", struct.pack("<d", d["energy"][c]))[0], c
+        want16 = struct.unpack("<I", struct.pack("<f", d["energy_bf16"][c]))[0] if prog == "pic" else 0
+        assert e16[c] == want16, c
+    for r in range(px * py):
+        assert ranks[r] == (int(d["counts"][r]), int(d["part_hash"][r]), int(d["rho_hash"][r])), r
+    tp = [float(x) for x in next(l for l in out.splitlines() if l.startswith("phases")).split()[1:]]
+    assert len(tp) == 7 and tp[6] > 0 and min(tp) >= 0 and sum(tp[:6]) <= tp[6] * 1.0001, tp
+
+
+def test_sim_pic_has_no_unordered_accesses(sim):
+    rc, out = _run_race([os.path.join(sim, "run_pic"), "3", "3", "8", "8", "300", "3", "7", "128"])
+    assert rc == 0, out[-3000:]
+    assert "simt race:" not in out and "race check: 0 report(s)" in out, out[-3000:]
