@@ -255,6 +255,35 @@ __device__ inline bool all_aligned16(CollCtx& cc, int word_a, int word_b, int on
     return ballot(bad) == 0;
 }
 
+// Elements [lo, hi) of a reduction with a user-defined operator, in the
+// reference's order (mpi.cu:209-225): acc = x_0, then fn(x_r, acc) for
+// r = 1 .. P-1, each call as fn(invec, inoutvec, len, datatype).  The fold
+// runs in a rank-heap buffer (an input may be an output in place) with the
+// slice split across the wave's lanes, each calling fn on its own run of
+// elements; the result is then copied to the root's output, or to every
+// member's.  Members' buffers are the table's published words.
+__device__ inline int user_fold_slice(CollCtx& cc, MPI_User_function* fn, MPI_Datatype dt, int es, uint64_t lo,
+                                      uint64_t hi, bool all, int root) {
+    const uint64_t n = hi - lo, bytes = n * (uint64_t)es;
+    uint8_t* acc = (uint8_t*)rank_malloc(bytes);
+    if (!acc) return MPI_ERR_NO_MEM;
+    rank_copy(acc, (const uint8_t*)(uintptr_t)peer_word(cc, 0, SW_SBUF) + lo * es, bytes);
+    const uint64_t per = (n + 63) / 64;
+    const uint64_t e0 = (uint64_t)lane_id() * per < n ? (uint64_t)lane_id() * per : n;
+    const uint64_t e1 = e0 + per < n ? e0 + per : n;
+    for (int r = 1; r < cc.size; r++) {
+        const uint8_t* x = (const uint8_t*)(uintptr_t)peer_word(cc, r, SW_SBUF) + lo * es;
+        int len = (int)(e1 - e0);
+        MPI_Datatype d = dt;
+        if (len > 0) fn((void*)(x + e0 * es), acc + e0 * es, &len, &d);
+        wave_sync();
+    }
+    for (int m = all ? 0 : root; m < (all ? cc.size : root + 1); m++)
+        rank_copy((uint8_t*)(uintptr_t)peer_word(cc, m, SW_RBUF) + lo * es, acc, bytes);
+    rank_free(acc, bytes);
+    return MPI_SUCCESS;
+}
+
 __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int count, MPI_Datatype dt, MPI_Op op,
                                     int root, MPI_Comm comm) {
     CollCtx cc;
@@ -265,15 +294,17 @@ __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int coun
     if (count < 0) return GPUMPI_FAIL(MPI_ERR_COUNT, "negative count in reduction");
     const uintptr_t opv = (uintptr_t)op;
     const bool builtin = opv >= 1 && opv <= GPUMPI_NUM_BUILTIN_OPS;
-    if (!builtin) return GPUMPI_FAIL(MPI_ERR_OP, "user-defined operators are not supported in reductions");
-    const int opi = (int)opv - 1;
-    if (!op_allowed(dt, opi)) return GPUMPI_FAIL(MPI_ERR_OP, "combination of datatype and operator not allowed");
+    if (op == MPI_OP_NULL) return GPUMPI_FAIL(MPI_ERR_OP, "MPI_OP_NULL in a reduction");
+    MPI_User_function* ufn = builtin ? nullptr : ((const ::MPI_Op_impl*)op)->fn;
+    if (!builtin && !ufn) return GPUMPI_FAIL(MPI_ERR_OP, "user-defined operator without a function");
+    const int opi = builtin ? (int)opv - 1 : -1;
+    if (builtin && !op_allowed(dt, opi)) return GPUMPI_FAIL(MPI_ERR_OP, "combination of datatype and operator not allowed");
     const uint64_t bytes = (uint64_t)count * (uint64_t)es;
     const bool all = root < 0;
     const bool i_out = all || cc.me == root;
     const void* src = (sendbuf == MPI_IN_PLACE) ? recvbuf : sendbuf;
     if (cc.size == 1) {
-        if (src != recvbuf) rank_copy(recvbuf, src, bytes);
+        if (src != recvbuf && (all || cc.me == root)) rank_copy(recvbuf, src, bytes);
         return MPI_SUCCESS;
     }
     CollStage stg;
@@ -290,7 +321,10 @@ __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int coun
     const uint64_t g1 = groups * (uint64_t)(cc.me + 1) / (uint64_t)cc.size;
     uint64_t lo = g0 * V, hi = g1 * V;
     if (hi > (uint64_t)count) hi = (uint64_t)count;
-    if (lo < hi) {
+    if (lo < hi && ufn) {
+        err = user_fold_slice(cc, ufn, dt, es, lo, hi, all, root);
+        if (err) return err;
+    } else if (lo < hi) {
         Cmd c{};
         c.op = CMD_REDUCE;
         c.i0 = dt;

@@ -19,14 +19,57 @@ namespace gpumpi {
 namespace rt {
 
 // ---------------------------------------------------------------------------
-// Rank heap (gpumpi_malloc): a bump allocator over the rank's arena slice.
-// free() of the most recent block rewinds; other frees are no-ops.
+// Rank heap (gpumpi_malloc / gpumpi_free) over the rank's arena slice, in
+// blocks of multiples of 256 B.  Free blocks form a list in address order
+// (each holds its size and the next free block in its first 16 bytes) and
+// merge with free neighbours; a free block that ends at the top lowers the
+// top instead.  Allocation takes the first free block that fits, splitting
+// off the rest, else raises the top.  Only the rank's program wave calls
+// these: one lane walks the list and the result is made wave-uniform.
 // ---------------------------------------------------------------------------
+struct HeapFree {
+    uint64_t bytes;
+    uint64_t next;  // offset + 1 of the next free block, 0 at the end
+};
+
+__device__ __forceinline__ uint64_t heap_need(uint64_t bytes) {
+    return bytes ? (bytes + 255) & ~(uint64_t)255 : 256;
+}
+
 __device__ inline void* rank_malloc(uint64_t bytes) {
     World* w = world();
     const uint64_t cap = w->cfg.arena_bytes;
     uint64_t* top = &w->arena_top[my_rank()];
-    const uint64_t need = (bytes + 255) & ~(uint64_t)255;
+    uint8_t* base = w->arena + (uint64_t)my_rank() * cap;
+    const uint64_t need = heap_need(bytes);
+    uint64_t got = 0;  // offset + 1 of a free block taken, 0 for none
+    if (elected()) {
+        RankLds* L = ctx();
+        uint64_t prev = 0, cur = L->heap_free;
+        while (cur) {
+            HeapFree* f = (HeapFree*)(base + cur - 1);
+            if (f->bytes >= need) {
+                uint64_t next = f->next;
+                if (f->bytes > need) {  // the upper part stays free
+                    HeapFree* g = (HeapFree*)(base + cur - 1 + need);
+                    g->bytes = f->bytes - need;
+                    g->next = next;
+                    next = cur + need;
+                }
+                if (prev) ((HeapFree*)(base + prev - 1))->next = next;
+                else L->heap_free = next;
+                got = cur;
+                break;
+            }
+            prev = cur;
+            cur = f->next;
+        }
+    }
+    got = uniform_u64(got);
+    if (got) {
+        wave_sync();
+        return base + got - 1;
+    }
     // One lane's view of the top, one lane's store: the block is the same
     // for every lane of the program's wave however its lanes are scheduled.
     const uint64_t off = uniform_u64(*top);
@@ -34,20 +77,67 @@ __device__ inline void* rank_malloc(uint64_t bytes) {
         GPUMPI_FAIL(MPI_ERR_NO_MEM, "rank heap exhausted");
         return nullptr;
     }
-    uint8_t* base = w->arena + (uint64_t)my_rank() * cap;
     if (elected()) *top = off + need;
     wave_sync();
     return base + off;
 }
 
+// Return a block of `bytes` (as passed to rank_malloc) at p.  A block that is
+// not inside the rank's allocated heap, or that overlaps a free block (freed
+// twice), raises MPI_ERR_ARG and is left alone.
 __device__ inline void rank_free(void* p, uint64_t bytes) {
+    if (!p) return;
     World* w = world();
     uint8_t* base = w->arena + (uint64_t)my_rank() * w->cfg.arena_bytes;
-    uint64_t* top = &w->arena_top[my_rank()];
-    const uint64_t need = (bytes + 255) & ~(uint64_t)255;
-    const uint64_t t = uniform_u64(*top);
-    if ((uint8_t*)p + need == base + t && elected()) *top = (uint64_t)((uint8_t*)p - base);
+    uint64_t* topp = &w->arena_top[my_rank()];
+    const uint64_t need = heap_need(bytes);
+    const uint64_t top = uniform_u64(*topp);
+    const uint64_t off = (uint64_t)((uint8_t*)p - base);
+    if ((uint8_t*)p < base || (off & 255) || off + need > top) {
+        GPUMPI_FAIL(MPI_ERR_ARG, "gpumpi_free: block outside the rank heap");
+        return;
+    }
+    int bad = 0;
+    if (elected()) {
+        RankLds* L = ctx();
+        auto F = [&](uint64_t o1) { return (HeapFree*)(base + o1 - 1); };
+        uint64_t pp = 0, prev = 0, cur = L->heap_free;
+        while (cur && cur - 1 < off) {
+            pp = prev;
+            prev = cur;
+            cur = F(cur)->next;
+        }
+        if ((prev && prev - 1 + F(prev)->bytes > off) || (cur && off + need > cur - 1)) {
+            bad = 1;
+        } else {
+            uint64_t start, start_pred;
+            if (prev && prev - 1 + F(prev)->bytes == off) {  // extends the free block below
+                F(prev)->bytes += need;
+                start = prev;
+                start_pred = pp;
+            } else {
+                F(off + 1)->bytes = need;
+                F(off + 1)->next = cur;
+                if (prev) F(prev)->next = off + 1;
+                else L->heap_free = off + 1;
+                start = off + 1;
+                start_pred = prev;
+            }
+            HeapFree* s = F(start);
+            if (cur && start - 1 + s->bytes == cur - 1) {  // and the one above
+                s->bytes += F(cur)->bytes;
+                s->next = F(cur)->next;
+            }
+            if (start - 1 + s->bytes == top) {  // ends at the top: lower it
+                if (start_pred) F(start_pred)->next = s->next;
+                else L->heap_free = s->next;
+                *topp = start - 1;
+            }
+        }
+    }
+    bad = uniform_i32(bad);
     wave_sync();
+    if (bad) GPUMPI_FAIL(MPI_ERR_ARG, "gpumpi_free: block already free");
 }
 
 // ---------------------------------------------------------------------------
@@ -88,17 +178,121 @@ __device__ __forceinline__ bool comm_valid(MPI_Comm c) { return c != MPI_COMM_NU
 // colour ordered by key, ties by parent rank) and context-id uniqueness of
 // communicator.cu:37-46 (a global counter instead of Allreduce(MAX)).
 // ---------------------------------------------------------------------------
-__device__ inline uint8_t* comm_pool_alloc(uint64_t bytes) {
+// The pool's lock guards its free list; only communicator creation and
+// MPI_Comm_free take it.  One lane of the calling wave holds it.
+__device__ inline bool pool_lock(PoolHead* h) {
+    Spinner sp;
+    for (;;) {
+        int got = 0;
+        if (elected()) {
+            int expect = 0;
+            got = __hip_atomic_compare_exchange_strong(&h->lock, &expect, 1, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) ? 1 : 0;
+        }
+        if (uniform_i32(got)) return true;
+        if (!sp.keep_going()) return false;
+    }
+}
+__device__ inline void pool_unlock(PoolHead* h) {
+    if (elected()) __hip_atomic_store(&h->lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    wave_sync();
+}
+
+// A block of at least `bytes` (multiples of 256 B): the first freed block
+// that fits (the part beyond `bytes` split off and kept free when it is 4 KiB
+// or more), else the top of the pool is raised.  *got is the block's size.
+__device__ inline uint8_t* comm_pool_alloc(uint64_t bytes, uint64_t* got) {
     World* w = world();
     bytes = (bytes + 255) & ~(uint64_t)255;
-    uint64_t off = 0;
-    if (elected()) off = __hip_atomic_fetch_add(w->comm_pool_top, bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    off = uniform_u64(off);
-    if (off + bytes > w->cfg.comm_pool_bytes) {
-        GPUMPI_FAIL(MPI_ERR_NO_MEM, "communicator pool exhausted");
-        return nullptr;
+    PoolHead* h = (PoolHead*)w->comm_pool;
+    if (!pool_lock(h)) return nullptr;
+    uint64_t off = 0, size = bytes;
+    if (elected()) {
+        uint64_t prev = 0, cur = h->free_head;
+        while (cur) {
+            PoolFree* f = (PoolFree*)(w->comm_pool + cur);
+            if (f->bytes >= bytes) {
+                uint64_t next = f->next;
+                size = f->bytes;
+                if (size - bytes >= 4096) {
+                    PoolFree* g = (PoolFree*)(w->comm_pool + cur + bytes);
+                    g->bytes = size - bytes;
+                    g->next = next;
+                    next = cur + bytes;
+                    size = bytes;
+                }
+                if (prev) ((PoolFree*)(w->comm_pool + prev))->next = next;
+                else h->free_head = next;
+                off = cur;
+                break;
+            }
+            prev = cur;
+            cur = f->next;
+        }
     }
+    pool_unlock(h);
+    off = uniform_u64(off);
+    size = uniform_u64(size);
+    if (!off) {
+        if (elected()) off = __hip_atomic_fetch_add(w->comm_pool_top, bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        off = uniform_u64(off);
+        size = bytes;
+        if (off + bytes > w->cfg.comm_pool_bytes) {
+            GPUMPI_FAIL(MPI_ERR_NO_MEM, "communicator pool exhausted");
+            return nullptr;
+        }
+    }
+    *got = size;
     return w->comm_pool + off;
+}
+
+// Put a communicator's block back on the pool's free list.
+__device__ inline void comm_pool_free(uint8_t* mem, uint64_t bytes) {
+    World* w = world();
+    PoolHead* h = (PoolHead*)w->comm_pool;
+    if (!pool_lock(h)) return;
+    if (elected()) {
+        PoolFree* f = (PoolFree*)mem;
+        f->bytes = bytes;
+        f->next = h->free_head;
+        h->free_head = (uint64_t)(mem - w->comm_pool);
+    }
+    pool_unlock(h);
+}
+
+// A fresh context id: the counter's next one while it lasts (3 .. kMaxCtx),
+// then one that MPI_Comm_free returned; -1 when none is left.
+__device__ inline int ctx_alloc() {
+    World* w = world();
+    int id = 0;
+    if (elected()) id = __hip_atomic_fetch_add(w->next_ctx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    id = uniform_i32(id);
+    if (id <= kMaxCtx) return id;
+    uint32_t* bits = (uint32_t*)(w->comm_pool + kPoolCtxBitsOff);
+    id = -1;
+    if (elected()) {
+        for (int wi = 0; wi < (kMaxCtx + 1) / 32 && id < 0; wi++) {
+            uint32_t v = __hip_atomic_load(&bits[wi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (v) {
+                const uint32_t bit = v & (0u - v);
+                const uint32_t old =
+                    __hip_atomic_fetch_and(&bits[wi], ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (old & bit) {
+                    id = wi * 32 + __builtin_ctz(bit);
+                    break;
+                }
+                v = old & ~bit;
+            }
+        }
+    }
+    return uniform_i32(id);
+}
+
+__device__ inline void ctx_free(int id) {
+    if (id < 3 || id > kMaxCtx) return;
+    uint32_t* bits = (uint32_t*)(world()->comm_pool + kPoolCtxBitsOff);
+    if (elected()) __hip_atomic_fetch_or(&bits[id >> 5], 1u << (id & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wave_sync();
 }
 
 // Split `parent` (this rank has rank `prank` in it) by (color, key).
@@ -141,21 +335,24 @@ __device__ inline MPI_Comm comm_split_impl(CommShared* parent, int prank, int co
     const int leader = uniform_i32(lrank);
     const int P = ctx()->nranks;
     if (color != MPI_UNDEFINED && new_rank == 0) {
-        World* w = world();
         const uint64_t bytes = sizeof(CommShared) + sizeof(int) * (uint64_t)new_size +
                                sizeof(int) * (uint64_t)P + sizeof(CollSlot) * (uint64_t)new_size + 512;
-        uint8_t* mem = comm_pool_alloc(bytes);
+        uint64_t got = 0;
+        uint8_t* mem = comm_pool_alloc(bytes, &got);
+        const int ctxid = mem ? ctx_alloc() : -1;
+        if (mem && ctxid < 0) {
+            comm_pool_free(mem, got);
+            GPUMPI_FAIL(MPI_ERR_COMM, "communicator context ids exhausted");
+            mem = nullptr;
+        }
         if (mem) {
             CommShared* nc = (CommShared*)mem;
             int* world_of = (int*)(mem + sizeof(CommShared));
             int* rank_of_world = world_of + new_size;
             CollSlot* slots = (CollSlot*)(((uintptr_t)(rank_of_world + P) + 255) & ~(uintptr_t)255);
             for (int i = lane_id(); i < P; i += 64) rank_of_world[i] = -1;
-            int ctxid = 0;
-            if (elected()) ctxid = __hip_atomic_fetch_add(w->next_ctx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ctxid = uniform_i32(ctxid);
-            if (ctxid > 32767) GPUMPI_FAIL(MPI_ERR_COMM, "communicator context ids exhausted");
             nc->bar = 0;
+            nc->pool_bytes = got;
             nc->ctx = ctxid;
             nc->size = new_size;
             nc->kind = 2;

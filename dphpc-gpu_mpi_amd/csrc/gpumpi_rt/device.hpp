@@ -203,6 +203,7 @@ struct RankLds {
     uint64_t bounce_top; // staging bump offset for private rendezvous sends
     int bounce_users;
     int detached;        // requests freed by MPI_Request_free that are still active
+    uint64_t heap_free;  // first free block of the rank heap: its offset + 1, 0 for none
     // offsets of the variable arrays from the LDS base
     uint32_t off_reqs, off_ring, off_rx, off_tx, off_unmatched, off_freebits, off_user;
     uint32_t lds_bytes;

@@ -64,10 +64,16 @@ __device__ inline void rank_init(World* w) {
         for (uint32_t i = threadIdx.x; i < ntx; i += blockDim.x) tx[i] = src[nrx + i];
         if (threadIdx.x == 0) ctx()->stamp = *(const uint32_t*)sv;
     }
+    if (rank == 0 && threadIdx.x < 64) {
+        // The communicator pool's allocator state starts empty (wave 0's own
+        // stores, ordered before any use by MPI_Init's barrier).
+        uint32_t* head = (uint32_t*)w->comm_pool;
+        for (uint32_t i = threadIdx.x; i < kPoolHeadBytes / 4; i += 64) head[i] = 0;
+    }
     if (threadIdx.x == 0) {
         w->arena_top[rank] = 0;
         if (rank == 0) {  // ordered before any use by MPI_Init's barrier
-            __hip_atomic_store(w->comm_pool_top, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(w->comm_pool_top, kPoolHeadBytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(w->next_ctx, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }

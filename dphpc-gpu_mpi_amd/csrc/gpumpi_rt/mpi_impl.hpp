@@ -157,6 +157,7 @@ __device__ inline int MPI_Op_create(MPI_User_function* fn, int commute, MPI_Op* 
     return MPI_SUCCESS;
 }
 __device__ inline int MPI_Op_free(MPI_Op* op) {
+    if ((uintptr_t)*op > GPUMPI_NUM_BUILTIN_OPS) gpumpi::rt::rank_free(*op, sizeof(MPI_Op_impl));
     *op = MPI_OP_NULL;
     return MPI_SUCCESS;
 }
@@ -513,8 +514,50 @@ __device__ inline int MPI_Comm_dup(MPI_Comm comm, MPI_Comm* newcomm) {
     *newcomm = comm_split_impl(c, r, 0, r);
     return MPI_SUCCESS;
 }
+// MPI_Comm_free (communicator.cu:80-94 frees on the last reference; freeing
+// MPI_COMM_WORLD or MPI_COMM_NULL only nulls the handle).  Collective over the
+// members: each completes its own requests on the communicator first (their
+// status sources are translated now, while the rank maps exist), then
+// counts itself out on the communicator's barrier word; the last member out
+// returns the block to the pool and the context id to the free ids.
 __device__ inline int MPI_Comm_free(MPI_Comm* comm) {
-    *comm = MPI_COMM_NULL;  // communicator memory lives until the launch ends
+    using namespace gpumpi::rt;
+    WaveJoin join_;
+    const MPI_Comm h = *comm;
+    *comm = MPI_COMM_NULL;
+    if (h == MPI_COMM_NULL || h == MPI_COMM_WORLD || h == MPI_COMM_SELF) return MPI_SUCCESS;
+    CommShared* c = comm_get(h);
+    if (c->kind != 2) return MPI_SUCCESS;
+    RankLds* L = ctx();
+    Spinner sp;
+    for (;;) {
+        bool pending = false;
+        for (int idx = 0; idx < L->R; idx++) {
+            Req* q = &reqs()[idx];
+            if (q->kind == REQ_FREE || (MPI_Comm)(uintptr_t)q->comm != h) continue;
+            if (q->state != RS_COMPLETE) {
+                pending = true;
+                continue;
+            }
+            if (q->kind == REQ_RECV && q->st_source >= 0) q->st_source = comm_rank_of_world(c, q->st_source);
+            q->comm = (uint64_t)(uintptr_t)MPI_COMM_WORLD;  // nothing left to translate
+        }
+        wave_sync();
+        if (!pending) break;
+        progress();
+        if (!sp.keep_going()) return MPI_ERR_OTHER;
+    }
+    const int size = c->size, id = c->ctx;
+    const uint64_t bytes = c->pool_bytes;
+    fence_release_agent();
+    uint64_t old = 0;
+    if (elected()) old = __hip_atomic_fetch_add(&c->bar, (uint64_t)1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    old = uniform_u64(old);
+    if ((old + 1) % (uint64_t)size == 0) {  // the last member out
+        fence_acquire_agent();
+        comm_pool_free((uint8_t*)c, bytes);
+        ctx_free(id);
+    }
     return MPI_SUCCESS;
 }
 
@@ -602,6 +645,9 @@ __device__ inline int MPI_Group_range_incl(MPI_Group group, int n, int ranges[][
     return MPI_SUCCESS;
 }
 __device__ inline int MPI_Group_free(MPI_Group* group) {
+    const MPI_Group g = *group;
+    if (g != MPI_GROUP_NULL && g != MPI_GROUP_WORLD && g != MPI_GROUP_EMPTY)
+        gpumpi::rt::rank_free(g, sizeof(int) * (uint64_t)(g->size + 2));
     *group = MPI_GROUP_NULL;
     return MPI_SUCCESS;
 }

@@ -99,7 +99,30 @@ struct alignas(128) CommShared {
     int* world_of;       // [size]: comm rank -> world rank
     int* rank_of_world;  // [world size]: world rank -> comm rank or -1
     CollSlot* slots;     // [size]: collective pointer table
+    uint64_t pool_bytes; // derived: size of its block in the communicator pool
 };
+
+// The communicator pool (World::comm_pool) starts with its allocator's state:
+// a lock, the list of blocks MPI_Comm_free returned, and at kPoolCtxBitsOff a
+// bitmap of the context ids it returned (32768 bits).  Blocks follow.
+struct PoolHead {
+    int lock;
+    int pad;
+    uint64_t free_head;  // offset of the first free block, 0 for none
+};
+struct PoolFree {
+    uint64_t bytes;
+    uint64_t next;  // offset of the next free block, 0 at the end
+};
+constexpr uint64_t kPoolCtxBitsOff = 4096;
+constexpr uint64_t kPoolHeadBytes = 8192;
+// Context ids are 15-bit (3 .. kMaxCtx; 0-2 are NULL, WORLD and SELF).  A
+// build may lower the limit (GPUMPI_MAX_CTX) to exercise id reuse quickly.
+#ifndef GPUMPI_MAX_CTX
+#define GPUMPI_MAX_CTX 32767
+#endif
+constexpr int kMaxCtx = GPUMPI_MAX_CTX;
+static_assert(kMaxCtx >= 3 && kMaxCtx <= 32767 && (kMaxCtx + 1) % 32 == 0, "context id limit");
 
 struct ErrorRecord {
     int code;      // first error's MPI error code

@@ -283,6 +283,7 @@ int build_config(const gpumpi_config* in, Config* out, int* lds_bytes) {
         return GPUMPI_EINVAL;
     }
     if (c.num_procs > 32767) { set_err("at most 32767 ranks"); return GPUMPI_EINVAL; }
+    if (c.comm_pool_bytes < (64u << 10)) { set_err("comm_pool_bytes must be at least 64 KiB"); return GPUMPI_EINVAL; }
     Config& g = *out;
     g = Config{};
     g.nranks = c.num_procs;

@@ -4,12 +4,13 @@
 #   build.sh OUTDIR DRIVER.cpp [DRIVER.cpp ...]   -> OUTDIR/<driver name>
 # SIMT_PROGS (optional): space-separated program sources (progs/) to build
 # instead of every progs/*.hip file.
+# SIMT_CFLAGS (optional): extra compiler flags for every source.
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 ROOT=$(cd "$HERE/../../.." && pwd)
 OUT=$1; shift
 CXX=${CXX:-/opt/rocm/llvm/bin/clang++}
-FLAGS="-std=c++17 -g -O1 -fno-omit-frame-pointer -ffp-contract=off -fsanitize=address,undefined -fno-sanitize=alignment,function -mllvm -asan-globals=0 -fsanitize-address-use-after-return=never -I $HERE -I $ROOT/include -I $ROOT/dphpc-gpu_mpi_amd/csrc"
+FLAGS="-std=c++17 -g -O1 -fno-omit-frame-pointer -ffp-contract=off -fsanitize=address,undefined -fno-sanitize=alignment,function -mllvm -asan-globals=0 -fsanitize-address-use-after-return=never -I $HERE -I $ROOT/include -I $ROOT/dphpc-gpu_mpi_amd/csrc $SIMT_CFLAGS"
 COV="-fsanitize-coverage=inline-8bit-counters,trace-loads,trace-stores"
 OBJ="$OUT/obj"
 mkdir -p "$OBJ"

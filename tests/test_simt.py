@@ -21,6 +21,9 @@ ENV = dict(os.environ, ASAN_OPTIONS="detect_odr_violation=0:detect_leaks=0", SIM
 REF_CPI = "/root/reference/toolchain_tests/pi/cpi.c"
 
 
+SIM_MAX_CTX = 255
+
+
 @pytest.fixture(scope="module")
 def sim():
     # One build of every simulated program and driver: build.sh compiles all
@@ -29,7 +32,7 @@ def sim():
     import converter
     os.makedirs(OUT, exist_ok=True)
     drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp",
-                                              "run_pic.cpp")]
+                                              "run_pic.cpp", "run_lifetime.cpp", "run_user_op.cpp")]
     ring = os.path.join(OUT, "conv_ring.cpp")
     converter.convert(os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
     drivers.append(ring)
@@ -37,7 +40,11 @@ def sim():
         cpi = os.path.join(OUT, "conv_cpi.cpp")
         converter.convert(REF_CPI, cpi)
         drivers.append(cpi)
-    env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip")
+    # Context ids limited to 3..255 here, so that id reuse is reached in a few
+    # hundred communicator creations.
+    env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip lifetime.hip "
+               "user_ops.hip",
+               SIMT_CFLAGS=f"-DGPUMPI_MAX_CTX={SIM_MAX_CTX}")
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
 
@@ -286,3 +293,59 @@ def test_sim_pic_has_no_unordered_accesses(sim):
     rc, out = _run_race([os.path.join(sim, "run_pic"), "3", "3", "8", "8", "300", "3", "7", "128"])
     assert rc == 0, out[-3000:]
     assert "simt race:" not in out and "race check: 0 report(s)" in out, out[-3000:]
+
+
+def _lifetime(sim, mode, P, iters, pool=None, race=False):
+    cmd = [os.path.join(sim, "run_lifetime"), str(mode), str(P), str(iters), "64"] + ([str(pool)] if pool else [])
+    rc, out = (_run_race if race else _run)(cmd)
+    assert rc == 0, out[-3000:]
+    oks = [int(l.split()[2]) for l in out.splitlines() if l.startswith("ok ")]
+    outs = [int(x) for x in next(l for l in out.splitlines() if l.startswith("out ")).split()[1:]]
+    return oks, outs, out
+
+
+@pytest.mark.parametrize("P", [1, 3])
+def test_sim_rank_heap_reuses_merges_and_lowers_the_top(sim, P):
+    # Exact block positions after frees, splits, merges and a lowered top;
+    # then 2500 random allocations and frees (up to 16 blocks of up to 64 KiB
+    # live, each painted and checked) in a 2 MiB heap, which a bump
+    # allocator would exhaust; the heap is empty again at the end.
+    oks, outs, _ = _lifetime(sim, 0, P, 2500)
+    assert oks == [1] * P and outs[0] == 2500 and outs[1] >= 8
+
+
+@pytest.mark.parametrize("P", [2, 5])
+def test_sim_comm_free_returns_blocks_and_context_ids(sim, P):
+    # More communicator creations than there are context ids (3..255 in this
+    # build) and than a 64 KiB pool holds blocks for: every one is freed.
+    iters = SIM_MAX_CTX + 40 if P == 2 else 60
+    oks, outs, _ = _lifetime(sim, 1, P, iters, pool=64 << 10)
+    assert oks == [1] * P and outs[0] == iters
+
+
+def test_sim_comm_free_completes_pending_requests(sim):
+    oks, _, out = _lifetime(sim, 2, 4, 0, race=True)
+    assert oks == [1] * 4
+    assert "simt race:" not in out and "race check: 0 report(s)" in out, out[-3000:]
+
+
+def _user_op_expected_sum(P, n):
+    # x_0 + 2 * (x_1 + ... + x_{P-1}) per element, summed: the reference's
+    # fold (acc = x_0, then fn(x_r, acc) for r = 1 .. P-1) with
+    # fn: inout = 2 * in + inout.  The reversed fold would weight x_{P-1} once.
+    i = np.arange(n, dtype=np.int64)
+    x = lambda r: (r * 7919 + i * 104729) % 100003 - 50000  # noqa: E731
+    return int((x(0) + 2 * sum((x(r) for r in range(1, P)), np.zeros(n, dtype=np.int64))).sum())
+
+
+@pytest.mark.parametrize("P,n,root,in_place", [(1, 10, -1, 1), (2, 1, 1, 0), (3, 1000, -1, 0), (3, 1000, -1, 1),
+                                               (4, 63, 2, 1), (5, 4097, 0, 0), (6, 300, 5, 0)])
+def test_sim_user_defined_operators(sim, P, n, root, in_place):
+    # In the program: the user FP64 sum equals MPI_SUM bit for bit at every
+    # receiving rank, and the asymmetric int64 operator equals its closed form.
+    rc, out = _run([os.path.join(sim, "run_user_op"), str(P), str(n), str(root), str(in_place), "64"])
+    assert rc == 0, out[-3000:]
+    oks = [int(l.split()[2]) for l in out.splitlines() if l.startswith("ok ")]
+    assert oks == [1] * P, out[-2000:]
+    got = int(next(l for l in out.splitlines() if l.startswith("sum ")).split()[1])
+    assert got == _user_op_expected_sum(P, n)
