@@ -28,6 +28,8 @@ each with its own CPU baseline:
            1M particles per rank, 100 cycles: ms per cycle, phase split,
            communication share, HBM roofline on 48 B per particle per cycle,
            energies against the oracle; CPU baseline: the oracle's cycle.
+  autotune the launch-knob autotuner at 256 ranks: chosen knobs, objective
+           against the defaults', every trial.
 Each side configuration runs in a child process of its own (--config NAME).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -355,7 +357,23 @@ def pic_config(gm, np, threads, cpu):
     return out
 
 
-CONFIGS = ("pi", "p2p", "npb_is", "allreduce", "pic")
+def autotune_config(gm):
+    """The autotuner (gpumpi/tune.py) at 256 ranks: one coordinate-descent
+    pass over threads per rank, copy-split threshold, eager threshold and
+    channel slots, on the ring's geometric-mean step time over 8 B .. 16 MiB.
+    Reports the choices, the objective against the defaults', and every
+    trial."""
+    from gpumpi import tune
+    res = tune.tune_device(gm, 256, repeats=1, log=lambda m: print(f"[tune] {m}", file=sys.stderr, flush=True))
+    return {"ranks": 256, "sizes": res["sizes"], "chosen": res["best"],
+            "objective_us": round(res["objective"] * 1e6, 3),
+            "default_objective_us": round(res["default_objective"] * 1e6, 3),
+            "speedup_vs_default": round(res["speedup_vs_default"], 4), "seconds": res["seconds"],
+            "trials": [{"knobs": t["knobs"], "objective_us": None if t["objective"] is None
+                        else round(t["objective"] * 1e6, 3)} for t in res["trials"]]}
+
+
+CONFIGS = ("pi", "p2p", "npb_is", "allreduce", "pic", "autotune")
 
 
 def run_config(name, args):
@@ -383,6 +401,8 @@ def run_config(name, args):
             return allreduce_config(gm, np, args.threads, args.wire)
         if name == "pic":
             return pic_config(gm, np, args.threads, cpu)
+        if name == "autotune":
+            return autotune_config(gm)
         raise ValueError(name)
     finally:
         gm.release_world()

@@ -355,6 +355,58 @@ int gpumpi_default_config(gpumpi_config* cfg, int num_procs) {
     return GPUMPI_OK;
 }
 
+// Tuning file (written by the autotuner, gpumpi/tune.py): "key value" lines,
+// '#' comments; keys threads_per_rank, helper_min_bytes, eager_bytes,
+// chan_slots.  Values are checked as build_config checks them.
+int gpumpi_apply_tuning(gpumpi_config* cfg, const char* path) {
+    if (!cfg || !path) {
+        set_err("gpumpi_apply_tuning: null argument");
+        return GPUMPI_EINVAL;
+    }
+    FILE* f = std::fopen(path, "r");
+    if (!f) {
+        set_err("cannot open tuning file %s", path);
+        return GPUMPI_EINVAL;
+    }
+    gpumpi_config t = *cfg;
+    char line[256];
+    int lineno = 0, rc = GPUMPI_OK;
+    while (rc == GPUMPI_OK && std::fgets(line, sizeof line, f)) {
+        lineno++;
+        char* h = std::strchr(line, '#');
+        if (h) *h = 0;
+        char key[64];
+        long long v = 0;
+        char extra[8];
+        const int n = std::sscanf(line, "%63s %lld %7s", key, &v, extra);
+        if (n <= 0) continue;  // blank
+        if (n != 2) {
+            set_err("%s:%d: expected 'key value'", path, lineno);
+            rc = GPUMPI_EINVAL;
+        } else if (!std::strcmp(key, "threads_per_rank")) {
+            if (v < 64 || v > 1024 || v % 64) rc = GPUMPI_EINVAL;
+            t.threads_per_rank = (int)v;
+        } else if (!std::strcmp(key, "helper_min_bytes")) {
+            if (v < 0 || v > (1ll << 30)) rc = GPUMPI_EINVAL;
+            t.helper_min_bytes = (int)v;
+        } else if (!std::strcmp(key, "eager_bytes")) {
+            if (v < 0 || v > kMaxEagerBytes) rc = GPUMPI_EINVAL;
+            t.eager_bytes = (int)v;
+        } else if (!std::strcmp(key, "chan_slots")) {
+            if (v < 1 || v > kMaxChanSlots || !is_pow2((int)v)) rc = GPUMPI_EINVAL;
+            t.chan_slots = (int)v;
+        } else if (std::strcmp(key, "ranks") && std::strcmp(key, "device") && std::strcmp(key, "objective")) {
+            set_err("%s:%d: unknown key %s", path, lineno, key);
+            rc = GPUMPI_EINVAL;
+            continue;
+        }
+        if (rc != GPUMPI_OK && n == 2) set_err("%s:%d: bad value %lld for %s", path, lineno, v, key);
+    }
+    std::fclose(f);
+    if (rc == GPUMPI_OK) *cfg = t;
+    return rc;
+}
+
 int gpumpi_parse_args(int argc, char** argv, gpumpi_config* cfg) {
     int pos = -1;
     for (int i = 0; i < argc; i++)
@@ -372,6 +424,14 @@ int gpumpi_parse_args(int argc, char** argv, gpumpi_config* cfg) {
             return end && *end == 0;
         };
         long long v = 0;
+        if (!std::strcmp(a, "--tuning")) {
+            if (i + 1 >= argc) {
+                set_err("bad or missing value for %s", a);
+                return -2;
+            }
+            if (gpumpi_apply_tuning(cfg, argv[++i]) != GPUMPI_OK) return -2;
+            continue;
+        }
         if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
             std::printf("GPU MPI options (after ---gpumpi):\n"
                         "  -n N   number of ranks (one workgroup each)\n"
@@ -382,6 +442,8 @@ int gpumpi_parse_args(int argc, char** argv, gpumpi_config* cfg) {
                         "  -e R   pending requests per rank\n"
                         "  -q Q   slots per (src,dst) channel\n"
                         "  -x E   eager message bytes\n"
+                        "  -m M   bytes from which a copy is split across the rank's waves\n"
+                        "  --tuning FILE  apply an autotuner's choices\n"
                         "  -t S   watchdog seconds (0: none)\n");
             continue;
         }
@@ -398,6 +460,7 @@ int gpumpi_parse_args(int argc, char** argv, gpumpi_config* cfg) {
         else if (!std::strcmp(a, "-e") || !std::strcmp(a, "--pendingBufferSize")) cfg->max_requests = (int)v;
         else if (!std::strcmp(a, "-q")) cfg->chan_slots = (int)v;
         else if (!std::strcmp(a, "-x")) cfg->eager_bytes = (int)v;
+        else if (!std::strcmp(a, "-m")) cfg->helper_min_bytes = (int)v;
         else if (!std::strcmp(a, "-t")) cfg->timeout_s = (double)v;
         else {
             set_err("unknown gpumpi option %s", a);

@@ -39,7 +39,7 @@ CORE_SYMBOLS = ["gpumpi_default_config", "gpumpi_parse_args", "gpumpi_run", "gpu
                 "gpumpi_main",
                 "gpumpi_device_alloc", "gpumpi_device_free", "gpumpi_memcpy_htod",
                 "gpumpi_memcpy_dtoh", "gpumpi_memset", "gpumpi_max_ranks", "gpumpi_last_error",
-                "gpumpi_reset_world", "gpumpi_release_world", "gpumpi_service_poll"]
+                "gpumpi_reset_world", "gpumpi_release_world", "gpumpi_service_poll", "gpumpi_apply_tuning"]
 PROGS_SYMBOLS = ["gpumpi_prog_lookup", "gpumpi_prog_count", "gpumpi_prog_name",
                  "gpumpi_fill_payload", "gpumpi_verify_payload", "gpumpi_fill_normal",
                  "gpumpi_copy_calibration"]
@@ -95,6 +95,7 @@ def load_core():
         vp, sz = ctypes.c_void_p, ctypes.c_size_t
         lib.gpumpi_default_config.argtypes = [ctypes.POINTER(Config), ctypes.c_int]
         lib.gpumpi_parse_args.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(Config)]
+        lib.gpumpi_apply_tuning.argtypes = [ctypes.POINTER(Config), ctypes.c_char_p]
         lib.gpumpi_run.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                    vp, ctypes.POINTER(Result)]
         lib.gpumpi_run_graph.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
@@ -154,10 +155,14 @@ def programs():
     return [p.gpumpi_prog_name(i).decode() for i in range(p.gpumpi_prog_count())]
 
 
-def default_config(nranks, **kw):
+def default_config(nranks, tuning=None, **kw):
+    """gpumpi_default_config, then a tuning file's choices (gpumpi_apply_tuning,
+    written by gpumpi.tune), then the keyword overrides."""
     core = load_core()
     c = Config()
     core.gpumpi_default_config(ctypes.byref(c), nranks)
+    if tuning is not None and core.gpumpi_apply_tuning(ctypes.byref(c), os.fsencode(tuning)) != 0:
+        raise GpumpiError(f"tuning file {tuning}: {core.gpumpi_last_error().decode()}")
     for k, v in kw.items():
         if not hasattr(c, k):
             raise KeyError(k)

@@ -78,6 +78,14 @@ int gpumpi_default_config(gpumpi_config* cfg, int num_procs);
  * marker is missing, -2 on a parse error; `cfg` is updated in place. */
 int gpumpi_parse_args(int argc, char** argv, gpumpi_config* cfg);
 
+/* Apply a tuning file written by the autotuner (dphpc-gpu_mpi_amd/gpumpi/
+ * tune.py): "key value" lines for threads_per_rank, helper_min_bytes,
+ * eager_bytes and chan_slots.  `cfg` changes only if every line is valid;
+ * returns GPUMPI_OK or GPUMPI_EINVAL (gpumpi_last_error says why).  Also
+ * reachable as `---gpumpi --tuning FILE`.  No counterpart in the reference:
+ * its launch knobs are the fixed CLI options of main.cu:70-78. */
+int gpumpi_apply_tuning(gpumpi_config* cfg, const char* path);
+
 /* Launch `kernel` (a gpumpi_rank_kernel<main> instance) with cfg->num_procs
  * ranks and wait for it.  `argc/argv` are copied to device memory and given
  * to every rank's program; `user_args` (a device pointer or NULL) is

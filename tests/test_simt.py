@@ -32,7 +32,8 @@ def sim():
     import converter
     os.makedirs(OUT, exist_ok=True)
     drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp",
-                                              "run_pic.cpp", "run_lifetime.cpp", "run_user_op.cpp")]
+                                              "run_pic.cpp", "run_lifetime.cpp", "run_user_op.cpp",
+                                              "run_p2p.cpp")]
     ring = os.path.join(OUT, "conv_ring.cpp")
     converter.convert(os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
     drivers.append(ring)
@@ -43,7 +44,7 @@ def sim():
     # Context ids limited to 3..255 here, so that id reuse is reached in a few
     # hundred communicator creations.
     env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip lifetime.hip "
-               "user_ops.hip",
+               "user_ops.hip p2p_bench.hip",
                SIMT_CFLAGS=f"-DGPUMPI_MAX_CTX={SIM_MAX_CTX}")
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
@@ -349,3 +350,19 @@ def test_sim_user_defined_operators(sim, P, n, root, in_place):
     assert oks == [1] * P, out[-2000:]
     got = int(next(l for l in out.splitlines() if l.startswith("sum ")).split()[1])
     assert got == _user_op_expected_sum(P, n)
+
+
+@pytest.mark.parametrize("case", [("ring", 4, 100000, 256, 448, 16384, 8), ("ring", 3, 5000, 128, 192, 4096, 4),
+                                  ("pingpong", 4, 3000, 64, 4032, 262144, 16), ("ring", 5, 70000, 192, 1984, 65536, 4),
+                                  ("pingpong", 2, 1000, 512, 960, 4096, 16), ("ring", 4, 200, 64, 192, 4096, 4),
+                                  ("ring", 3, 4032, 128, 4032, 65536, 16)])
+def test_sim_p2p_under_every_tuner_candidate(sim, case):
+    # The autotuner (gpumpi/tune.py) launches the ring under each candidate
+    # of threads per rank, copy-split threshold, eager threshold and channel
+    # slots; every candidate value here moves the payloads intact, on both
+    # sides of the eager threshold.
+    prog, P, nbytes, threads, eager, helper, slots = case
+    rc, out = _run([os.path.join(sim, "run_p2p"), prog, str(P), str(nbytes), str(threads), str(eager), str(helper),
+                    str(slots)])
+    assert rc == 0, out[-3000:]
+    assert "mismatches 0" in out, out[-2000:]
