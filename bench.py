@@ -19,8 +19,8 @@ each with its own CPU baseline:
   p2p      8 B ping-pong of one pair and of 128 concurrent pairs, and the
            8 B - 64 MiB ping-pong sweep (one pair, 128 pairs);
   npb_is   NPB IS class B at 128 ranks: keys/s, Alltoallv GB/s and roofline,
-           per-phase times; CPU baseline: the oracle's restatement of IS
-           (one host thread);
+           per-phase times; CPU baseline: the reference's is.c under host MPI
+           at 128 ranks (else the oracle's restatement on one host thread);
   allreduce  the exact FP32 Allreduce of 256 ranks x 16 MiB; with --wire
            also the compressed-wire Allreduce (BF16, FP8 e4m3) against the
            exact path at 256 ranks x 1M floats, with their error against FP64;
@@ -131,8 +131,26 @@ def cpi_host_baseline(P=64, n=100000000, runs=3):
 
 
 def is_host_baseline():
-    """The oracle's restatement of NPB IS class B on 128 simulated ranks, one
-    host thread; keys/s over its 10 timed iterations."""
+    """NPB IS class B at 128 ranks on the host: the reference's own is.c
+    under host MPI (oracle/_ref/npb/is.B.x, built by `make -C oracle ref`;
+    its reported time and Mop/s over the 10 timed iterations, is.c:1148),
+    else the oracle's restatement on one host thread."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "npb", "is.B.x")
+    mpirun = "/opt/conda/bin/mpirun"
+    if os.path.exists(exe) and os.path.exists(mpirun):
+        out = subprocess.run([mpirun, "-np", "128", exe], capture_output=True, text=True, timeout=600).stdout
+        vals = {}
+        for line in out.splitlines():
+            if "=" in line:
+                k, _, v = line.partition("=")
+                vals[k.strip()] = v.strip()
+        if vals.get("Verification") == "SUCCESSFUL" and "Time in seconds" in vals:
+            t = float(vals["Time in seconds"])
+            cores = min(os.cpu_count() or 1, 16)
+            return {"value": round(10 * (1 << 25) / t / 1e6, 1), "unit": "Mkeys/s", "cores": cores,
+                    "kind": "reference",
+                    "sample": f"the reference's NPB IS (is.c) class B under host MPI, 128 ranks on {cores} cores "
+                              f"(oversubscribed), 10 timed iterations: {t:.2f} s, verification SUCCESSFUL"}
     for p in (os.path.join(ROOT, "oracle"),):
         if p not in sys.path:
             sys.path.insert(0, p)

@@ -318,3 +318,22 @@ def test_pic_ballistic_closed_form(orc):
     assert np.allclose(d["energy"], e0, rtol=1e-13, atol=0)
     vel = lambda q: np.sort(q[:, 2:].view(np.uint64).ravel())
     assert np.array_equal(vel(d["particles"]), vel(init))
+
+
+REF_IS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle", "_ref", "npb")
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+@pytest.mark.skipif(not (os.path.exists(os.path.join(REF_IS, "is.S.x")) and os.path.exists(MPIRUN)),
+                    reason="the reference's NPB IS is built by `make -C oracle ref` where the reference is present")
+@pytest.mark.parametrize("cls,P", [("S", 1), ("S", 4), ("S", 16), ("B", 8)])
+def test_reference_npb_is_verifies_where_the_oracle_does(orc, cls, P):
+    # The reference's own is.c under host MPI and the oracle's restatement
+    # pass NPB's known-answer verification for the same class and ranks.
+    import subprocess
+    out = subprocess.run([MPIRUN, "-np", str(P), os.path.join(REF_IS, f"is.{cls}.x")], capture_output=True,
+                         text=True, timeout=300).stdout
+    assert "Verification    =               SUCCESSFUL" in out, out[-1500:]
+    assert f"Size:  {33554432 if cls == 'B' else 65536}" in out.replace("  (class", " (class")
+    res, _, _ = orc.is_run(cls, P)
+    assert res.passed_verification == 5 * 10 + P
