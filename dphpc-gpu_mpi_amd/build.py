@@ -12,6 +12,7 @@ import argparse
 import concurrent.futures as cf
 import glob
 import os
+import shutil
 import subprocess
 import sys
 import time
@@ -90,20 +91,59 @@ def build(jobs=8, force=False, verbose=True):
     # executables with the reference's `---gpumpi` command line.  lib/ring is
     # this package's example; lib/cpi is the reference's own pi program,
     # converted from its source when that is at hand.
+    # lib/is.S.x and lib/is.B.x are the reference's NPB IS (is.c with
+    # common/c_print_results.c and common/c_timers.c, unmodified).
     import gpumpicc
-    examples = [(os.path.join(PKG, "examples", "ring.c"), os.path.join(LIB, "ring"))]
+    examples = [([os.path.join(PKG, "examples", "ring.c")], os.path.join(LIB, "ring"), [])]
     ref_cpi = os.path.join(REFERENCE, "toolchain_tests", "pi", "cpi.c")
     if os.path.exists(ref_cpi):
-        examples.append((ref_cpi, os.path.join(LIB, "cpi")))
+        examples.append(([ref_cpi], os.path.join(LIB, "cpi"), []))
+    if os.path.exists(os.path.join(NPB, "IS", "is.c")) and shutil.which("gcc"):
+        for cls in ("S", "B"):
+            srcs = [os.path.join(NPB, "IS", "is.c"), os.path.join(NPB, "common", "c_print_results.c"),
+                    os.path.join(NPB, "common", "c_timers.c")]
+            examples.append((srcs, os.path.join(LIB, f"is.{cls}.x"), [npb_params("is", cls)]))
     conv_deps = [core, os.path.join(PKG, "converter.py"), os.path.join(PKG, "gpumpicc.py")] + _headers()
-    for src, exe in examples:
-        if force or _stale(exe, [src] + conv_deps):
-            t0 = time.time()
-            gpumpicc.build([src], exe, workdir=OBJ, rpath="$ORIGIN")
-            if verbose:
-                print(f"[build] {os.path.basename(exe)} (converted {os.path.basename(src)}) {time.time() - t0:.1f}s",
-                      flush=True)
+
+    def convert_one(srcs, exe, incs):
+        if not (force or _stale(exe, srcs + conv_deps)):
+            return
+        t0 = time.time()
+        work = os.path.join(OBJ, "conv_" + os.path.basename(exe))
+        os.makedirs(work, exist_ok=True)
+        gpumpicc.build(srcs, exe, include_dirs=incs, workdir=work, rpath="$ORIGIN")
+        if verbose:
+            print(f"[build] {os.path.basename(exe)} (converted {', '.join(os.path.basename(x) for x in srcs)}) "
+                  f"{time.time() - t0:.1f}s", flush=True)
+
+    with cf.ThreadPoolExecutor(max_workers=min(jobs, len(examples))) as ex:
+        for f in [ex.submit(convert_one, *e) for e in examples]:
+            f.result()
     return core, progs
+
+
+NPB = os.path.join(REFERENCE, "toolchain_tests", "npb", "NPB3.4.1", "NPB3.4-MPI")
+
+
+def npb_params(bench, cls):
+    """Directory holding npbparams.h of NPB benchmark `bench`, class `cls`,
+    made by the reference's own procedure (toolchain_tests/npb/is_test.sh:
+    11-15): config/make.def is its make.def.template with MPICC set to this
+    package's gpumpicc, and its sys/setparams.c writes the header."""
+    d = os.path.join(OBJ, "npb")
+    os.makedirs(os.path.join(d, "config"), exist_ok=True)
+    with open(os.path.join(NPB, "config", "make.def.template")) as f:
+        tmpl = f.read()
+    with open(os.path.join(d, "config", "make.def"), "w") as f:
+        f.write(tmpl.replace("MPICC = mpicc", "MPICC = " + os.path.join(PKG, "gpumpicc.py")))
+    sp = os.path.join(d, "setparams")
+    if not os.path.exists(sp):
+        subprocess.run(["gcc", "-O2", "-o", sp, os.path.join(NPB, "sys", "setparams.c")], check=True,
+                       capture_output=True)
+    wd = os.path.join(d, f"{bench}_{cls}")
+    os.makedirs(wd, exist_ok=True)
+    subprocess.run([sp, bench, cls], cwd=wd, check=True, stdout=subprocess.DEVNULL)
+    return wd
 
 
 def main():

@@ -176,7 +176,39 @@ def collect(ast, main_file):
                 edits.append((loc[1], 0, "main", loc[1] + loc[2]))
         if k in ("VarDecl", "ParmVarDecl", "FunctionDecl") and in_main(loc) and n.get("name") in CXX_KEYWORDS:
             edits.append((loc[1], 1, "insert", "__decl_"))
+    # Prototypes in the program's own headers (not converted themselves) get
+    # a device redeclaration after the main file's last #include: clang
+    # overloads on the target, so device code calls the device function.
+    decls = []
+    for n, parents, loc, b, e in nodes:
+        if (n.get("kind") != "FunctionDecl" or n.get("isImplicit") or not loc or not loc[0] or in_main(loc)
+                or not b or not e or b[0] != loc[0] or e[0] != loc[0] or not user_header(loc[0])
+                or not parents or parents[-1].get("kind") != "TranslationUnitDecl"
+                or any(c.get("kind") == "CompoundStmt" for c in n.get("inner", []))):
+            continue
+        with open(loc[0], "rb") as f:
+            text = f.read()[b[1]:e[1] + e[2]].decode()
+        d = "__device__ " + " ".join(text.split()) + ";"
+        if d not in decls:
+            decls.append(d)
+    if decls:
+        with open(main_file, "rb") as f:
+            data = f.read()
+        at = 0
+        for m in re.finditer(rb"^[ \t]*#[ \t]*include[^\n]*\n", data, re.M):
+            at = m.end()
+        edits.append((at, -1, "insert", "\n".join(decls) + "\n"))
     return edits
+
+
+SYSTEM_PREFIXES = ("/usr/", "/opt/", "/lib/", os.path.join(ROOT, "include") + os.sep, tempfile.gettempdir() + os.sep)
+
+
+def user_header(path):
+    """A header of the program's own (not the C library's, not this
+    runtime's, not the generated C view of mpi.h)."""
+    real = os.path.realpath(path)
+    return not any(real.startswith(pfx) for pfx in SYSTEM_PREFIXES)
 
 
 def apply(text, edits):

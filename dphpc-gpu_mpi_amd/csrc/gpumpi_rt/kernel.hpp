@@ -104,6 +104,19 @@ __device__ inline void rank_save(World* w) {
     ((uint32_t*)sv)[1] = kSaveMagic;
 }
 
+// End the calling rank's program now, as if it returned `code`: the rank's
+// exit work (return value, channel cursors, helper waves released) and then
+// the program wave itself ends.  Called by the program wave.
+[[noreturn]] __device__ inline void rank_exit(int code) {
+    World* w = world();
+    if (elected()) w->rank_ret[my_rank()] = code;
+    wave_sync();
+    rank_save(w);
+    helpers_exit();
+    fence_release_agent();
+    __builtin_amdgcn_endpgm();
+}
+
 template <int (*Main)(int, char**)>
 __device__ inline void rank_body(World* w, int argc, char** argv) {
     rank_init(w);
@@ -141,6 +154,9 @@ __device__ inline int __gpu_max_threads() { return GPU_MPI_MAX_RANKS; }
 // Rank heap, wave-uniform (every lane gets the same pointer).
 __device__ inline void* gpumpi_malloc(size_t bytes) { return gpumpi::rt::rank_malloc(bytes); }
 __device__ inline void gpumpi_free(void* p, size_t bytes) { gpumpi::rt::rank_free(p, bytes); }
+// End this rank's program with return value `code` (C's exit in converted
+// programs).
+[[noreturn]] __device__ inline void gpumpi_rank_exit(int code) { gpumpi::rt::rank_exit(code); }
 // The opaque argument block handed to the launch (host C-ABI user_args).
 template <typename T>
 __device__ inline T* gpumpi_user_args() { return (T*)gpumpi::rt::world()->user_args; }

@@ -72,6 +72,10 @@ def build(sources, output, include_dirs=(), extra=(), workdir=None, verbose=Fals
         cmd += ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc")]
         for d in include_dirs:
             cmd += ["-I", d]
+        # the converted copies live elsewhere: quoted includes relative to a
+        # source's own directory are found through it
+        for d in dict.fromkeys(os.path.dirname(os.path.abspath(x)) for x in sources):
+            cmd += ["-I", d]
         cmd += ["-x", "hip"] + converted + ["-o", output, "-L", lib, "-lgpumpi", "-Wl,-rpath," + (rpath or lib)]
         cmd += list(extra)
         if verbose:

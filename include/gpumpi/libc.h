@@ -13,6 +13,7 @@
  *   atoi atol atoll atof strtol  strlen strcmp strncmp strcpy strncpy strcat
  *   strchr memcmp memcpy memset  malloc calloc realloc free  abs labs
  *   rand srand (a per-rank 48-bit LCG, not the host's sequence)
+ *   exit (ends the calling rank's program)  getenv (always NULL)
  */
 #ifndef GPUMPI_LIBC_H
 #define GPUMPI_LIBC_H
@@ -187,6 +188,13 @@ __device__ inline void* gpumpi_c_realloc(void* p, size_t n) {
     }
     return q;
 }
+// exit(): the calling rank's program ends here with `code` as its return
+// value (gpumpi_rank_exit), as the reference's __gpu_exit ended the rank's
+// thread (gpu_libs/gpu_libc/stdlib.cu:50-53).  Other ranks are not told:
+// programs call MPI_Abort first where they mean the whole job.
+__device__ inline void gpumpi_c_exit(int code) { gpumpi_rank_exit(code); }
+// getenv(): the device has no environment.
+__device__ inline char* gpumpi_c_getenv(const char*) { return nullptr; }
 __device__ inline int gpumpi_c_abs(int v) { return v < 0 ? -v : v; }
 __device__ inline long gpumpi_c_labs(long v) { return v < 0 ? -v : v; }
 
@@ -231,6 +239,8 @@ __device__ inline int gpumpi_c_rand() {
 #define calloc gpumpi_c_calloc
 #define realloc gpumpi_c_realloc
 #define free gpumpi_c_free
+#define exit gpumpi_c_exit
+#define getenv gpumpi_c_getenv
 #define abs gpumpi_c_abs
 #define labs gpumpi_c_labs
 #define rand gpumpi_c_rand

@@ -1,7 +1,7 @@
 #!/bin/bash
 # Builds the CPU simulation of the device runtime, the host launcher and rank
 # programs, and links one executable per scenario driver:
-#   build.sh OUTDIR DRIVER.cpp [DRIVER.cpp ...]   -> OUTDIR/<driver name>
+#   build.sh OUTDIR DRIVER.cpp[:MORE.cpp...] [DRIVER.cpp ...]   -> OUTDIR/<driver name>
 # SIMT_PROGS (optional): space-separated program sources (progs/) to build
 # instead of every progs/*.hip file.
 # SIMT_CFLAGS (optional): extra compiler flags for every source.
@@ -22,7 +22,9 @@ else
 fi
 pids=""
 objs=""
-for s in $common "$@"; do
+# A driver may be several sources joined by ':' (a program of several
+# translation units); the executable is named after the first.
+for s in $common $(echo "$@" | tr ':' ' '); do
   o="$OBJ/$(basename $s).o"
   $CXX $FLAGS $COV -x c++ -c "$s" -o "$o" &
   pids="$pids $!"
@@ -33,5 +35,8 @@ pids="$pids $!"
 for p in $pids; do wait $p; done
 for s in $common; do objs="$objs $OBJ/$(basename $s).o"; done
 for d in "$@"; do
-  $CXX $FLAGS $objs "$OBJ/simt.cpp.o" "$OBJ/$(basename $d).o" -o "$OUT/$(basename $d .cpp)"
+  dobjs=""
+  for part in $(echo "$d" | tr ':' ' '); do dobjs="$dobjs $OBJ/$(basename $part).o"; done
+  first=$(echo "$d" | cut -d: -f1)
+  $CXX $FLAGS $objs "$OBJ/simt.cpp.o" $dobjs -o "$OUT/$(basename $first .cpp)"
 done

@@ -67,6 +67,8 @@ bool is_private(const void* p);
 bool is_shared(const void* p);
 uint64_t steady_counter();
 [[noreturn]] void fail(const char* what);
+// The calling lane's fiber ends here, as the lane's part of s_endpgm.
+[[noreturn]] void lane_exit();
 // Happens-before checker hooks (no-ops unless SIMT_RACE=1).
 enum RaceKind { RACE_LOAD = 0, RACE_STORE = 1, RACE_RMW = 2 };
 void race_fence(int order, const char* scope);
@@ -117,6 +119,7 @@ inline T simt_readfirstlane(T v) { return simt_from<T>(::simt::wave_op(::simt::O
 #define __builtin_amdgcn_readlane(v, l) __shfl((v), (l))
 #define __builtin_amdgcn_fence(order, scope) ::simt::race_fence((order), (scope))
 #define __builtin_amdgcn_s_sleep(n) ::simt::yield()
+#define __builtin_amdgcn_endpgm() ::simt::lane_exit()
 #define __builtin_amdgcn_wave_barrier() ((void)::simt::wave_op(::simt::OP_SYNC, 0, 0))
 inline unsigned simt_alignbyte(unsigned hi, unsigned lo, unsigned r) {
     return (unsigned)(((((unsigned long long)hi) << 32) | lo) >> (8 * (r & 3)));

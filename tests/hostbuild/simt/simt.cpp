@@ -126,15 +126,22 @@ static void run_kernel_body() {
     ((RankKernel)g_kernel)(*(void**)g_args[0], *(int*)g_args[1], *(char***)g_args[2]);
 }
 
-extern "C" void simt_fiber_entry() {
-    __sanitizer_finish_switch_fiber(nullptr, nullptr, nullptr);
-    run_kernel_body();
+[[noreturn]] static void finish_lane() {
     cur->done = true;
     cur->wave->done++;
     wave_check(*cur->wave);
     switch_to(sched_fiber, true);
     fail("finished fiber resumed");
 }
+
+extern "C" void simt_fiber_entry() {
+    __sanitizer_finish_switch_fiber(nullptr, nullptr, nullptr);
+    run_kernel_body();
+    finish_lane();
+}
+
+// s_endpgm: the lane's fiber ends without returning through its frames.
+void lane_exit() { finish_lane(); }
 
 // Completes the pending wave operation once every live lane has arrived at
 // it, or releases the lanes waiting at the memory-step barrier once every

@@ -2,7 +2,8 @@
 of include/gpumpi/gpu_main.h (gpumpi_main) with the reference's `---gpumpi`
 command line (gpu_libs/gpu_main/main.cu:48-97), text through the host
 service channel.  lib/ring is dphpc-gpu_mpi_amd/examples/ring.c, lib/cpi the
-reference's toolchain_tests/pi/cpi.c, both built by converter.py and
+reference's toolchain_tests/pi/cpi.c, lib/is.S.x and lib/is.B.x its NPB IS, all
+built by converter.py and
 gpumpicc.py (dphpc-gpu_mpi_amd/build.py)."""
 import json
 import os
@@ -48,3 +49,20 @@ def test_converted_reference_cpi(orc):
     assert pi == float(f"{want:.16f}")
     host = float(json.load(open(os.path.join(GOLDEN, "cpi_mpich.json")))["pi_printed"]["64"])
     assert abs(pi - host) <= 1e-13 * abs(host)
+
+
+@pytest.mark.parametrize("cls,P,heap", [("S", 1, 64 << 20), ("S", 4, 256 << 20), ("S", 16, 1 << 30),
+                                        ("B", 128, 16 << 30)])
+def test_converted_reference_npb_is(orc, cls, P, heap):
+    # The reference's NPB IS (is.c, c_print_results.c, c_timers.c, unmodified)
+    # converted into lib/is.<class>.x: NPB's own verification passes, as it
+    # does for is.c under host MPI and for the oracle's restatement.
+    exe = os.path.join(LIB, f"is.{cls}.x")
+    if not os.path.exists(exe):
+        pytest.skip(f"lib/is.{cls}.x is not built (the reference's NPB IS was not at hand)")
+    p = _run([exe, "---gpumpi", "-n", str(P), "-b", "256", "-p", str(heap)])
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert re.search(r"Verification\s+=\s+SUCCESSFUL", p.stdout), p.stdout[-2000:]
+    assert re.search(rf"Total processes\s+=\s+{P}\b", p.stdout)
+    res, _, _ = orc.is_run(cls, P)
+    assert res.passed_verification == 5 * 10 + P

@@ -6,8 +6,10 @@ wires), the compressed-wire Allreduce, the uneven Alltoallv and the block collec
 GPU tests, and a launch in which one rank aborts while the others keep
 calling collectives.  With SIMT_RACE=1 the simulation also checks every
 cross-rank access for happens-before ordering (simt.cpp)."""
+import importlib.util
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -366,3 +368,36 @@ def test_sim_p2p_under_every_tuner_candidate(sim, case):
                     str(slots)])
     assert rc == 0, out[-3000:]
     assert "mismatches 0" in out, out[-2000:]
+
+
+NPB_IS = "/root/reference/toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI"
+
+
+@pytest.mark.skipif(not os.environ.get("GPUMPI_SIM_SLOW"),
+                    reason="about 6 minutes: set GPUMPI_SIM_SLOW=1 to run the converted NPB IS in the simulation")
+@pytest.mark.skipif(not os.path.exists(os.path.join(NPB_IS, "IS", "is.c")), reason="the reference's NPB IS is absent")
+def test_sim_reference_npb_is_converts_and_verifies(tmp_path):
+    # The reference's is.c, c_print_results.c and c_timers.c, unmodified,
+    # through the converter; class S on one rank passes NPB's verification.
+    # (More ranks reach full_verify's MPI_Irecv into a stack variable, which
+    # the simulation does not model: a lane's private address differs per
+    # lane here, while on the GPU one private address names every lane's own.)
+    import converter
+    spec = importlib.util.spec_from_file_location("gpumpi_build", os.path.join(HERE, "..", "dphpc-gpu_mpi_amd",
+                                                                               "build.py"))
+    pkg_build = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(pkg_build)
+    inc = pkg_build.npb_params("is", "S")
+    parts = []
+    for src in ("IS/is.c", "common/c_print_results.c", "common/c_timers.c"):
+        dst = str(tmp_path / ("conv_" + os.path.basename(src).replace(".c", ".cpp")))
+        converter.convert(os.path.join(NPB_IS, src), dst, [inc])
+        parts.append(dst)
+    env = dict(os.environ, SIMT_PROGS="io.hip",
+               SIMT_CFLAGS=f"-I {NPB_IS}/IS -I {NPB_IS}/common -I {inc}")
+    out = str(tmp_path / "out")
+    subprocess.run(["bash", os.path.join(SIM, "build.sh"), out, ":".join(parts)], check=True, env=env, timeout=1200)
+    rc, text = _run([os.path.join(out, "conv_is"), "---gpumpi", "-n", "1", "-b", "64", "-p", str(4 << 20)],
+                    timeout=1200)
+    assert rc == 0, text[-3000:]
+    assert "Verification    =               SUCCESSFUL" in text, text[-3000:]
