@@ -92,7 +92,8 @@ def build(jobs=8, force=False, verbose=True):
     # this package's example; lib/cpi is the reference's own pi program,
     # converted from its source when that is at hand.
     # lib/is.S.x and lib/is.B.x are the reference's NPB IS (is.c with
-    # common/c_print_results.c and common/c_timers.c, unmodified).
+    # common/c_print_results.c and common/c_timers.c, unmodified), lib/dt.S.x
+    # its NPB DT (dt.c, DGraph.c and the common files, unmodified).
     import gpumpicc
     examples = [([os.path.join(PKG, "examples", "ring.c")], os.path.join(LIB, "ring"), [])]
     ref_cpi = os.path.join(REFERENCE, "toolchain_tests", "pi", "cpi.c")
@@ -103,6 +104,9 @@ def build(jobs=8, force=False, verbose=True):
             srcs = [os.path.join(NPB, "IS", "is.c"), os.path.join(NPB, "common", "c_print_results.c"),
                     os.path.join(NPB, "common", "c_timers.c")]
             examples.append((srcs, os.path.join(LIB, f"is.{cls}.x"), [npb_params("is", cls)]))
+        dt = [os.path.join(NPB, "DT", "dt.c"), os.path.join(NPB, "DT", "DGraph.c")] + \
+            [os.path.join(NPB, "common", c) for c in ("c_print_results.c", "c_timers.c", "randdp.c")]
+        examples.append((dt, os.path.join(LIB, "dt.S.x"), [npb_params("dt", "S")]))
     conv_deps = [core, os.path.join(PKG, "converter.py"), os.path.join(PKG, "gpumpicc.py")] + _headers()
 
     def convert_one(srcs, exe, incs):

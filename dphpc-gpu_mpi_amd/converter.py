@@ -111,7 +111,7 @@ class Locator:
         return None
 
 
-def collect(ast, main_file):
+def collect(ast, main_file, view_dir=None):
     """Edits as (offset, order, kind, payload) on the main file."""
     main_real = os.path.realpath(main_file)
     lc = Locator()
@@ -174,6 +174,10 @@ def collect(ast, main_file):
             edits.append((b[1], 0, "insert", "__device__ "))
             if n.get("name") == "main":
                 edits.append((loc[1], 0, "main", loc[1] + loc[2]))
+            else:
+                kr = kr_parameters(n, loc, main_file)
+                if kr:
+                    edits.append(kr)
         if k in ("VarDecl", "ParmVarDecl", "FunctionDecl") and in_main(loc) and n.get("name") in CXX_KEYWORDS:
             edits.append((loc[1], 1, "insert", "__decl_"))
     # Prototypes in the program's own headers (not converted themselves) get
@@ -182,7 +186,7 @@ def collect(ast, main_file):
     decls = []
     for n, parents, loc, b, e in nodes:
         if (n.get("kind") != "FunctionDecl" or n.get("isImplicit") or not loc or not loc[0] or in_main(loc)
-                or not b or not e or b[0] != loc[0] or e[0] != loc[0] or not user_header(loc[0])
+                or not b or not e or b[0] != loc[0] or e[0] != loc[0] or not user_header(loc[0], view_dir)
                 or not parents or parents[-1].get("kind") != "TranslationUnitDecl"
                 or any(c.get("kind") == "CompoundStmt" for c in n.get("inner", []))):
             continue
@@ -201,14 +205,54 @@ def collect(ast, main_file):
     return edits
 
 
-SYSTEM_PREFIXES = ("/usr/", "/opt/", "/lib/", os.path.join(ROOT, "include") + os.sep, tempfile.gettempdir() + os.sep)
+SYSTEM_PREFIXES = ("/usr/", "/opt/", "/lib/", os.path.join(ROOT, "include") + os.sep)
 
 
-def user_header(path):
-    """A header of the program's own (not the C library's, not this
-    runtime's, not the generated C view of mpi.h)."""
+def user_header(path, view_dir=None):
+    """A header of the program's own: not the C library's, not this
+    runtime's, not the generated C view of mpi.h (in `view_dir`)."""
     real = os.path.realpath(path)
-    return not any(real.startswith(pfx) for pfx in SYSTEM_PREFIXES)
+    pfx = SYSTEM_PREFIXES + ((os.path.realpath(view_dir) + os.sep,) if view_dir else ())
+    return not any(real.startswith(p) for p in pfx)
+
+
+def kr_parameters(fn, loc, main_file):
+    """An old-style (K&R) definition -- `double f(X, A) double *X; double *A; {`
+    -- as a prototype: the edit replacing everything from its parameter
+    list to the body with `(double *X, double *A) ` (the reference's
+    converter fixed these too)."""
+    body = next((c for c in fn.get("inner", []) if c.get("kind") == "CompoundStmt"), None)
+    if body is None:
+        return None
+    bb = body.get("range", {}).get("begin", {})
+    if "offset" not in bb or "spellingLoc" in bb:
+        return None
+    with open(main_file, "rb") as f:
+        data = f.read()
+    start, stop = loc[1] + loc[2], bb["offset"]
+    seg = data[start:stop].decode(errors="replace")
+    i = seg.find("(")
+    if i < 0:
+        return None
+    depth, j = 0, i
+    while j < len(seg):
+        depth += seg[j] == "("
+        depth -= seg[j] == ")"
+        if depth == 0:
+            break
+        j += 1
+    rest = re.sub(r"/\*.*?\*/|//[^\n]*", "", seg[j + 1:], flags=re.S).strip()
+    if not rest:
+        return None  # a prototype already
+    params = []
+    for p in fn.get("inner", []):
+        if p.get("kind") != "ParmVarDecl":
+            continue
+        t, name = p.get("type", {}).get("qualType", "int"), p.get("name", "")
+        if "(" in t:
+            return None  # function-pointer parameters stay as written
+        params.append(f"{t} {name}")  # array parameters come adjusted to pointers
+    return (start + i, 0, "replace", (stop, "(" + ", ".join(params) + ") "))
 
 
 def apply(text, edits):
@@ -222,6 +266,10 @@ def apply(text, edits):
         pos = off
         if kind == "insert":
             out.append(payload.encode())
+        elif kind == "replace":
+            end, new = payload
+            out.append(new.encode())
+            pos = end
         elif kind == "init":
             end = payload
             init = data[off:end].decode()
@@ -277,7 +325,7 @@ def convert(src, dst, include_dirs=()):
         text = f.read()
     with tempfile.TemporaryDirectory() as tmp:
         ast = parse_ast(src, list(include_dirs), tmp)
-    edits = collect(ast, src)
+        edits = collect(ast, src, os.path.join(tmp, "include"))
     has_main = defines_main(ast, src)
     body = apply(text, edits)
     header = (f"// Converted from {os.path.basename(src)} for the device MPI runtime (dphpc-gpu_mpi_amd/"

@@ -11,7 +11,8 @@
  * to have the standard names mean these.
  *
  *   atoi atol atoll atof strtol  strlen strcmp strncmp strcpy strncpy strcat
- *   strchr memcmp memcpy memset  malloc calloc realloc free  abs labs
+ *   strchr strrchr strstr strdup memcmp memcpy memmove memset  malloc calloc
+ *   realloc free  abs labs
  *   rand srand (a per-rank 48-bit LCG, not the host's sequence)
  *   exit (ends the calling rank's program)  getenv (always NULL)
  */
@@ -165,6 +166,22 @@ __device__ inline char* gpumpi_c_strchr(const char* s, int c) {
         if (!*s) return nullptr;
     }
 }
+__device__ inline char* gpumpi_c_strrchr(const char* s, int c) {
+    const char* last = nullptr;
+    for (;; s++) {
+        if (*s == (char)c) last = s;
+        if (!*s) return (char*)last;
+    }
+}
+__device__ inline char* gpumpi_c_strstr(const char* h, const char* n) {
+    if (!*n) return (char*)h;
+    for (; *h; h++) {
+        size_t i = 0;
+        while (n[i] && h[i] == n[i]) i++;
+        if (!n[i]) return (char*)h;
+    }
+    return nullptr;
+}
 __device__ inline int gpumpi_c_memcmp(const void* a, const void* b, size_t n) {
     const unsigned char *x = (const unsigned char*)a, *y = (const unsigned char*)b;
     for (size_t i = 0; i < n; i++)
@@ -172,6 +189,17 @@ __device__ inline int gpumpi_c_memcmp(const void* a, const void* b, size_t n) {
     return 0;
 }
 __device__ inline void* gpumpi_c_memcpy(void* d, const void* s, size_t n) { return gpumpi::libc::copy(d, s, n); }
+// memmove: through a rank-heap buffer when the ranges overlap.
+__device__ inline void* gpumpi_c_memmove(void* d, const void* s, size_t n) {
+    const uint8_t *a = (const uint8_t*)s, *b = (const uint8_t*)d;
+    if (b + n <= a || a + n <= b || n == 0) return gpumpi::libc::copy(d, s, n);
+    void* t = gpumpi::libc::heap_alloc(n);
+    if (!t) return nullptr;
+    gpumpi::libc::copy(t, s, n);
+    gpumpi::libc::copy(d, t, n);
+    gpumpi::libc::heap_free(t);
+    return d;
+}
 __device__ inline void* gpumpi_c_memset(void* d, int c, size_t n) { return gpumpi::libc::fill(d, c, n); }
 __device__ inline void* gpumpi_c_malloc(size_t n) { return gpumpi::libc::heap_alloc(n); }
 __device__ inline void* gpumpi_c_calloc(size_t m, size_t n) {
@@ -185,8 +213,15 @@ __device__ inline void* gpumpi_c_realloc(void* p, size_t n) {
     if (p && q) {
         const size_t old = (size_t)*(const uint64_t*)((const uint8_t*)p - 16);
         gpumpi::libc::copy(q, p, old < n ? old : n);
+        gpumpi::libc::heap_free(p);
     }
     return q;
+}
+__device__ inline char* gpumpi_c_strdup(const char* s) {
+    const size_t n = gpumpi_c_strlen(s) + 1;
+    char* d = (char*)gpumpi::libc::heap_alloc(n);
+    if (d) gpumpi::libc::copy(d, s, n);
+    return d;
 }
 // exit(): the calling rank's program ends here with `code` as its return
 // value (gpumpi_rank_exit), as the reference's __gpu_exit ended the rank's
@@ -232,12 +267,16 @@ __device__ inline int gpumpi_c_rand() {
 #define strncpy gpumpi_c_strncpy
 #define strcat gpumpi_c_strcat
 #define strchr gpumpi_c_strchr
+#define strrchr gpumpi_c_strrchr
+#define strstr gpumpi_c_strstr
+#define memmove gpumpi_c_memmove
 #define memcmp gpumpi_c_memcmp
 #define memcpy gpumpi_c_memcpy
 #define memset gpumpi_c_memset
 #define malloc gpumpi_c_malloc
 #define calloc gpumpi_c_calloc
 #define realloc gpumpi_c_realloc
+#define strdup gpumpi_c_strdup
 #define free gpumpi_c_free
 #define exit gpumpi_c_exit
 #define getenv gpumpi_c_getenv

@@ -15,6 +15,9 @@ the pi fixture, the reference's cpi.c under /root/reference):
     scenarios (dphpc-gpu_mpi_amd/csrc/progs/semantics_scenarios.h) under
     host MPI (oracle/hostmpi/semantics_golden.c) at 4, 8 and 13 ranks: every
     rank's record.
+  * npb_dt_mpich.json -- the reference's NPB DT class S (oracle/Makefile
+    `ref`) under host MPI for the BH and WH graphs at 5 ranks and SH at 12:
+    the printed L2 norm and NPB's verification.
 """
 import json
 import os
@@ -62,6 +65,20 @@ def ring():
         json.dump(out, f, indent=1)
 
 
+def npb_dt():
+    run(["make", "-s", "-C", ORACLE, "_ref/npb/dt.S.x"])
+    exe = os.path.join(ORACLE, "_ref", "npb", "dt.S.x")
+    out = {}
+    for graph, np_ in (("BH", 5), ("WH", 5), ("SH", 12)):
+        txt = run([MPIRUN, "-np", str(np_), exe, graph], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                  text=True).stdout  # the norm is printed to stderr
+        norm = re.search(r"DT_\w+\.S L2 Norm = ([0-9.]+)", txt).group(1)
+        ok = re.search(r"Verification\s+=\s+SUCCESSFUL", txt) is not None
+        out[graph] = {"ranks": np_, "l2_norm": norm, "verified": ok}
+    with open(os.path.join(HERE, "npb_dt_mpich.json"), "w") as f:
+        json.dump({"source": "the reference's NPB DT class S under MPICH 3.3.2", "runs": out}, f, indent=1)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "semantics":
         semantics()
@@ -69,8 +86,12 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "ring":
         ring()
         return 0
+    if len(sys.argv) > 1 and sys.argv[1] == "npb_dt":
+        npb_dt()
+        return 0
     semantics()
     ring()
+    npb_dt()
     run(["make", "-C", ORACLE, "_build/hostmpi_reduce_golden", "ref"])
     run([MPIRUN, "-np", "5", os.path.join(ORACLE, "_build", "hostmpi_reduce_golden"),
          os.path.join(HERE, "reduce_mpich.bin")])

@@ -62,6 +62,7 @@ inline T gpumpi_host_readfirstlane(T) { gpumpi_host_unsupported("readfirstlane")
 #define __builtin_amdgcn_readlane(v, l) gpumpi_host_readfirstlane(v)
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 #define __builtin_amdgcn_s_sleep(n) ((void)0)
+#define __builtin_amdgcn_endpgm() std::abort()  // never reached by the host checks
 #define __builtin_amdgcn_wave_barrier() ((void)0)
 // v_alignbyte_b32: bytes [r, r+4) of the 8-byte concatenation lo:hi (hi is the upper half).
 inline unsigned gpumpi_host_alignbyte(unsigned hi, unsigned lo, unsigned r) {

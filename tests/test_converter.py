@@ -65,3 +65,37 @@ def test_parse_errors_are_reported(tmp_path):
     src.write_text("int main(void) { return undefined_thing; }\n")
     with pytest.raises(RuntimeError, match="parsing"):
         converter.convert(str(src), str(tmp_path / "bad.cpp"))
+
+
+def test_old_style_definitions_become_prototypes(tmp_path):
+    # K&R definitions (NPB's common/randdp.c is written so) get a prototype
+    # parameter list; an array parameter is its adjusted pointer type, as C
+    # defines it.
+    src = tmp_path / "kr.c"
+    src.write_text("double scale(x, n, v)\ndouble *x;\nint n;\ndouble v[4];\n{ return x[n] * v[0]; }\n"
+                   "int plain(int a) { return a; }\n")
+    out = tmp_path / "kr.cpp"
+    converter.convert(str(src), str(out))
+    t = out.read_text()
+    assert "__device__ double scale(double * x, int n, double * v) {" in t
+    assert "__device__ int plain(int a) { return a; }" in t
+
+
+def test_prototypes_of_own_headers_get_device_redeclarations(tmp_path):
+    # A program's own header is not rewritten; the converted file redeclares
+    # its functions for the device after the last #include, so calls into
+    # another converted file link (NPB's c_timers.h).
+    inc = tmp_path / "inc"
+    inc.mkdir()
+    (inc / "timers.h").write_text("#ifndef T_H\n#define T_H\nextern void   timer_start( int n );\n"
+                                  "double timer_read(int n);\n#endif\n")
+    src = tmp_path / "use.c"
+    src.write_text('#include <stdio.h>\n#include "timers.h"\nint f(void) { timer_start(1); '
+                   'return (int)timer_read(1); }\n')
+    out = tmp_path / "use.cpp"
+    converter.convert(str(src), str(out), [str(inc)])
+    t = out.read_text()
+    body = t.index("__device__ int f(void)")
+    for d in ("__device__ extern void timer_start( int n );", "__device__ double timer_read(int n);"):
+        assert d in t and t.index(d) < body, d
+    assert "printf" not in t.split('#include "timers.h"')[1].split("__device__ int f")[0]  # no libc redeclarations

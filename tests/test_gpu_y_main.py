@@ -2,9 +2,9 @@
 of include/gpumpi/gpu_main.h (gpumpi_main) with the reference's `---gpumpi`
 command line (gpu_libs/gpu_main/main.cu:48-97), text through the host
 service channel.  lib/ring is dphpc-gpu_mpi_amd/examples/ring.c, lib/cpi the
-reference's toolchain_tests/pi/cpi.c, lib/is.S.x and lib/is.B.x its NPB IS, all
-built by converter.py and
-gpumpicc.py (dphpc-gpu_mpi_amd/build.py)."""
+reference's toolchain_tests/pi/cpi.c, lib/is.S.x and lib/is.B.x its NPB IS,
+lib/dt.S.x its NPB DT, all built by converter.py and gpumpicc.py
+(dphpc-gpu_mpi_amd/build.py)."""
 import json
 import os
 import re
@@ -66,3 +66,20 @@ def test_converted_reference_npb_is(orc, cls, P, heap):
     assert re.search(rf"Total processes\s+=\s+{P}\b", p.stdout)
     res, _, _ = orc.is_run(cls, P)
     assert res.passed_verification == 5 * 10 + P
+
+
+@pytest.mark.parametrize("graph", ["BH", "WH", "SH"])
+def test_converted_reference_npb_dt(graph):
+    # The reference's NPB DT class S (dt.c, DGraph.c and the common files,
+    # unmodified) converted into lib/dt.S.x: the L2 norm host MPICH prints for
+    # the same graph and rank count (tests/golden/npb_dt_mpich.json), and
+    # NPB's verification.
+    exe = os.path.join(LIB, "dt.S.x")
+    if not os.path.exists(exe):
+        pytest.skip("lib/dt.S.x is not built (the reference's NPB DT was not at hand)")
+    gold = json.load(open(os.path.join(GOLDEN, "npb_dt_mpich.json")))["runs"][graph]
+    p = _run([exe, graph, "---gpumpi", "-n", str(gold["ranks"]), "-b", "256", "-p", str(256 << 20)])
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    text = p.stdout + p.stderr
+    assert f"DT_{graph}.S L2 Norm = {gold['l2_norm']}" in text, text[-2000:]
+    assert re.search(r"Verification\s+=\s+SUCCESSFUL", text)

@@ -24,6 +24,15 @@ REF_CPI = "/root/reference/toolchain_tests/pi/cpi.c"
 
 
 SIM_MAX_CTX = 255
+NPB = "/root/reference/toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI"
+
+
+def _pkg_build():
+    spec = importlib.util.spec_from_file_location("gpumpi_build", os.path.join(HERE, "..", "dphpc-gpu_mpi_amd",
+                                                                               "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
 
 
 @pytest.fixture(scope="module")
@@ -45,9 +54,19 @@ def sim():
         drivers.append(cpi)
     # Context ids limited to 3..255 here, so that id reuse is reached in a few
     # hundred communicator creations.
+    cflags = f"-DGPUMPI_MAX_CTX={SIM_MAX_CTX}"
+    if os.path.exists(os.path.join(NPB, "DT", "dt.c")):
+        # the reference's NPB DT, unmodified, as one converted program
+        inc = _pkg_build().npb_params("dt", "S")
+        parts = []
+        for src in ("DT/dt.c", "DT/DGraph.c", "common/c_print_results.c", "common/c_timers.c", "common/randdp.c"):
+            dst = os.path.join(OUT, "conv_" + os.path.basename(src).replace(".c", ".cpp"))
+            converter.convert(os.path.join(NPB, src), dst, [inc])
+            parts.append(dst)
+        drivers.append(":".join(parts))
+        cflags += f" -I {NPB}/DT -I {NPB}/common -I {inc}"
     env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip lifetime.hip "
-               "user_ops.hip p2p_bench.hip",
-               SIMT_CFLAGS=f"-DGPUMPI_MAX_CTX={SIM_MAX_CTX}")
+               "user_ops.hip p2p_bench.hip", SIMT_CFLAGS=cflags)
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
 
@@ -370,7 +389,7 @@ def test_sim_p2p_under_every_tuner_candidate(sim, case):
     assert "mismatches 0" in out, out[-2000:]
 
 
-NPB_IS = "/root/reference/toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI"
+NPB_IS = NPB
 
 
 @pytest.mark.skipif(not os.environ.get("GPUMPI_SIM_SLOW"),
@@ -383,11 +402,7 @@ def test_sim_reference_npb_is_converts_and_verifies(tmp_path):
     # the simulation does not model: a lane's private address differs per
     # lane here, while on the GPU one private address names every lane's own.)
     import converter
-    spec = importlib.util.spec_from_file_location("gpumpi_build", os.path.join(HERE, "..", "dphpc-gpu_mpi_amd",
-                                                                               "build.py"))
-    pkg_build = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(pkg_build)
-    inc = pkg_build.npb_params("is", "S")
+    inc = _pkg_build().npb_params("is", "S")
     parts = []
     for src in ("IS/is.c", "common/c_print_results.c", "common/c_timers.c"):
         dst = str(tmp_path / ("conv_" + os.path.basename(src).replace(".c", ".cpp")))
@@ -401,3 +416,18 @@ def test_sim_reference_npb_is_converts_and_verifies(tmp_path):
                     timeout=1200)
     assert rc == 0, text[-3000:]
     assert "Verification    =               SUCCESSFUL" in text, text[-3000:]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(NPB, "DT", "dt.c")), reason="the reference's NPB DT is absent")
+@pytest.mark.parametrize("graph", ["BH", "WH", "SH"])
+def test_sim_reference_npb_dt_matches_host_mpi(sim, graph):
+    # The reference's NPB DT class S (dt.c, DGraph.c and the common files,
+    # unmodified) through the converter: the L2 norm host MPICH prints for
+    # the same graph, and NPB's verification.
+    import json
+    gold = json.load(open(os.path.join(HERE, "golden", "npb_dt_mpich.json")))["runs"][graph]
+    rc, out = _run([os.path.join(sim, "conv_dt"), graph, "---gpumpi", "-n", str(gold["ranks"]), "-b", "64"]
+                   + SMALL_HEAP, timeout=1200)
+    assert rc == 0, out[-3000:]
+    assert f"DT_{graph}.S L2 Norm = {gold['l2_norm']}" in out, out[-3000:]
+    assert "Verification    =               SUCCESSFUL" in out
