@@ -54,3 +54,27 @@ def test_pic_phase_times_are_reported(gm):
     _, _, cnt, _, _, tp = run_pic(gm, 2, 2, 16, 16, 1000, 3, want_phases=True)
     assert int(cnt.sum()) == 4000
     assert tp[6] > 0 and all(t >= 0 for t in tp[:6]) and tp[:6].sum() <= tp[6] * 1.0001
+
+
+@pytest.mark.parametrize("shape,P", [((2, 2, 16, 16, 500, 5), 3), ((4, 4, 32, 32, 5000, 6), 16),
+                                     ((8, 8, 128, 128, 20000, 3), 64)])
+def test_particle_decomposition_pic_matches_the_global_oracle(gm, orc, shape, P):
+    # The reference's sputniPIC pattern (progs/pic_pdecomp.hip): field Bcast,
+    # particle Scatterv / Gatherv with uneven counts, FP32 Reduce of the
+    # counts, FP64 Reduce of the energy; against orc_pic_global.
+    px, py, nx, ny, np_, cycles = shape
+    n = px * py * np_
+    print(f"[gpu] pic_pdecomp {shape} P={P}", flush=True)
+    f = lambda x: struct.unpack("<I", struct.pack("<f", x))[0]  # noqa: E731
+    e = gm.DeviceBuffer(8 * cycles)
+    e.zero()
+    g = gm.DeviceBuffer(16 * n)
+    g.zero()
+    res = gm.run("pic_pdecomp", P, [0, px, py, nx, ny, np_, cycles, 2024, f(0.02), f(0.01), f(0.001), e, g, 0],
+                 heap_bytes=P * (16 * (n // P + 1) + 24 * px * nx * py * ny + (2 << 20)), timeout_s=120.0)
+    assert res.status == 0, res.describe()
+    eg, qg = orc.pic_global(orc.pic_params(*shape, seed=2024))
+    assert np.array_equal(g.download(np.float32).reshape(-1, 4).view(np.uint32),
+                          np.asarray(qg, dtype=np.float32).reshape(-1, 4).view(np.uint32))
+    got = e.download(np.float64)
+    assert np.max(np.abs(got - eg) / eg) <= 1e-13

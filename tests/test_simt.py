@@ -44,7 +44,7 @@ def sim():
     os.makedirs(OUT, exist_ok=True)
     drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp",
                                               "run_pic.cpp", "run_lifetime.cpp", "run_user_op.cpp",
-                                              "run_p2p.cpp")]
+                                              "run_p2p.cpp", "run_pic_pdecomp.cpp")]
     ring = os.path.join(OUT, "conv_ring.cpp")
     converter.convert(os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
     drivers.append(ring)
@@ -66,7 +66,7 @@ def sim():
         drivers.append(":".join(parts))
         cflags += f" -I {NPB}/DT -I {NPB}/common -I {inc}"
     env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip lifetime.hip "
-               "user_ops.hip p2p_bench.hip", SIMT_CFLAGS=cflags)
+               "user_ops.hip p2p_bench.hip pic_pdecomp.hip", SIMT_CFLAGS=cflags)
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
 
@@ -431,3 +431,28 @@ def test_sim_reference_npb_dt_matches_host_mpi(sim, graph):
     assert rc == 0, out[-3000:]
     assert f"DT_{graph}.S L2 Norm = {gold['l2_norm']}" in out, out[-3000:]
     assert "Verification    =               SUCCESSFUL" in out
+
+
+def _fnv1a(data):
+    h = 1469598103934665603
+    for x in data:
+        h = ((h ^ x) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+@pytest.mark.parametrize("shape,P", [((2, 2, 16, 16, 500, 5), 3), ((3, 2, 8, 12, 100, 7), 4),
+                                     ((1, 1, 10, 10, 64, 4), 5)])
+def test_sim_particle_decomposition_pic_matches_the_global_oracle(sim, orc, shape, P):
+    # sputniPIC's pattern (field Bcast, particle Scatterv / Gatherv with uneven
+    # counts, FP32 Reduce of the counts, FP64 Reduce of the energy) against
+    # orc_pic_global: the final particles bit for bit, in order; the energy
+    # within FP64 summation order.
+    import struct
+    rc, out = _run([os.path.join(sim, "run_pic_pdecomp")] + [str(v) for v in shape] + ["2024", str(P), "64"])
+    assert rc == 0, out[-3000:]
+    e = [struct.unpack("<d", struct.pack("<Q", int(l.split()[2])))[0] for l in out.splitlines()
+         if l.startswith("energy ")]
+    h = int(next(l for l in out.splitlines() if l.startswith("particles ")).split()[3])
+    eg, qg = orc.pic_global(orc.pic_params(*shape, seed=2024))
+    assert h == _fnv1a(np.asarray(qg, dtype=np.float32).tobytes())
+    assert len(e) == shape[5] and max(abs(a - b) / b for a, b in zip(e, eg)) <= 1e-13
