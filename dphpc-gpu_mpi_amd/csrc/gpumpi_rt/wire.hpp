@@ -172,11 +172,15 @@ __device__ __forceinline__ void wire_group(int fmt, const uint8_t* w, uint64_t n
     }
 }
 // Rank-order FP32 fold of the dequantised members over this member's
-// groups: worker t of nt takes whole 16-element groups, kFoldBatch
-// members' loads in flight, and writes the 16 sums to every member's
-// output.  The last, partial group goes element by element.
+// groups: worker t of nt takes whole 16-element groups, B members' loads in
+// flight, and writes the 16 sums to every member's output.  The last,
+// partial group goes element by element.
 __device__ inline void wire_fold_part(const WireFoldArgs& A, int t, int nt) {
-    constexpr int B = 4;  // members per batch (16 values each in flight)
+    // One member per batch keeps this body (called through the helper
+    // waves' indirect call) within the caller-saved registers, so it needs
+    // no stack frame: the kernel's fixed private segment is sized from its
+    // direct calls only, and a frame here would lie beyond it.
+    constexpr int B = 1;
     const int P = A.nmembers;
     const uint64_t full_groups = A.n / kWireGroup;
     for (uint64_t g = A.g0 + (uint64_t)t; g < A.g1; g += (uint64_t)nt) {
