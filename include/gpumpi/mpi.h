@@ -33,7 +33,12 @@
  *     view, as MPI specifies (the reference counted bytes); MPI_File_open
  *     does not truncate an existing file and MPI_MODE_EXCL fails only when
  *     the file exists; MPI_File_delete is local; read_at/write_at return the
- *     element count like read/write.
+ *     element count like read/write;
+ *   - MPI_Comm_free is collective over the communicator's members: each
+ *     completes its own requests on it first, and the last member out returns
+ *     its memory and context id (the reference deleted on the last local
+ *     reference); freeing MPI_COMM_WORLD, MPI_COMM_SELF or MPI_COMM_NULL only
+ *     nulls the handle, as in the reference.
  */
 #ifndef GPUMPI_MPI_H
 #define GPUMPI_MPI_H
