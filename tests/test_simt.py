@@ -54,6 +54,9 @@ def sim():
         drivers.append(cpi)
     # Context ids limited to 3..255 here, so that id reuse is reached in a few
     # hundred communicator creations.
+    exit_prog = os.path.join(OUT, "conv_exit.cpp")
+    converter.convert(os.path.join(HERE, "programs", "exit_early.c"), exit_prog)
+    drivers.append(exit_prog)
     cflags = f"-DGPUMPI_MAX_CTX={SIM_MAX_CTX}"
     if os.path.exists(os.path.join(NPB, "DT", "dt.c")):
         # the reference's NPB DT, unmodified, as one converted program
@@ -456,3 +459,15 @@ def test_sim_particle_decomposition_pic_matches_the_global_oracle(sim, orc, shap
     eg, qg = orc.pic_global(orc.pic_params(*shape, seed=2024))
     assert h == _fnv1a(np.asarray(qg, dtype=np.float32).tobytes())
     assert len(e) == shape[5] and max(abs(a - b) / b for a, b in zip(e, eg)) <= 1e-13
+
+
+def test_sim_converted_exit_ends_only_the_calling_rank(sim):
+    # exit() in a converted program ends the calling rank's program with that
+    # value (its exit work, then s_endpgm): the rest of that rank's main does
+    # not run, the other ranks finish, and the executable reports a rank
+    # that returned non-zero.
+    rc, out = _run([os.path.join(sim, "conv_exit"), "---gpumpi", "-n", "4", "-b", "64"] + SMALL_HEAP)
+    assert rc == 1, out[-3000:]
+    assert "exit_early: size 4 sum 6" in out and out.count("rank leaving with 3") == 1
+    printed = sorted(int(l.split(": ")[1]) for l in out.splitlines() if l.startswith("never printed by rank 1"))
+    assert printed == [0, 2, 3], out[-3000:]
