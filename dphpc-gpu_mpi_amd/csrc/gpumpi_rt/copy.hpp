@@ -9,6 +9,15 @@
 namespace gpumpi {
 namespace rt {
 
+// The calling lane's view of a private (scratch) address kept in shared
+// state (a request's buffer, say): on the GPU one private address names
+// every lane's own copy, so it is the address itself.  The CPU simulation of
+// the runtime (tests/hostbuild/simt) defines it to map the address into the
+// calling lane's stack.
+#ifndef GPUMPI_LANE_PRIVATE
+#define GPUMPI_LANE_PRIVATE(p) (p)
+#endif
+
 __device__ __forceinline__ bool is_private_ptr(const void* p) {
     return __builtin_amdgcn_is_private((const __attribute__((address_space(0))) void*)p);
 }

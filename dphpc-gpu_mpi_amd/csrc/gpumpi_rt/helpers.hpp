@@ -84,7 +84,7 @@ __device__ inline void par_run(const F& f, uint64_t bytes) {
 __device__ inline void rank_copy(void* dst, const void* src, uint64_t n) {
     if (n == 0) return;
     if (is_private_ptr(dst)) {
-        copy_all_lanes(dst, src, n);
+        copy_all_lanes(GPUMPI_LANE_PRIVATE(dst), src, n);
         return;
     }
     if (is_private_ptr(src) || n < (uint64_t)ctx()->helper_min_bytes || ctx()->waves == 1) {

@@ -370,10 +370,11 @@ __device__ inline void slot_take(int src, uint32_t seq, void* dstv, uint64_t cap
             const uint32_t val = slot_word_value(v);
             if (priv) {
                 // Every lane fills its own copy of a private buffer.
+                uint8_t* own = GPUMPI_LANE_PRIVATE(dst);
                 const int lim = need - base < 64 ? need - base : 64;
                 for (int j = 0; j < lim; j++) {
                     const uint32_t x = lane_u32(val, j);
-                    if (base + j >= SF_PAYLOAD) slot_payload_put(dst, n, (uint64_t)(base + j - SF_PAYLOAD), x);
+                    if (base + j >= SF_PAYLOAD) slot_payload_put(own, n, (uint64_t)(base + j - SF_PAYLOAD), x);
                 }
             } else {
                 const int w = base + lane_id();

@@ -47,7 +47,7 @@ __device__ inline void svc_release(SvcSlot* s) {
 __device__ inline void svc_copy(void* dst, const void* src, uint64_t n) {
     const int lane = lane_id();
     if (is_private_ptr(dst)) {
-        uint8_t* d = (uint8_t*)dst;
+        uint8_t* d = (uint8_t*)GPUMPI_LANE_PRIVATE(dst);
         const uint8_t* s = (const uint8_t*)src;
         for (uint64_t i = 0; i < n; i++) d[i] = s[i];
         wave_sync();
@@ -151,7 +151,8 @@ __device__ inline int svc_format(int stream, char* out, uint64_t cap, const char
         uint64_t n = cap - 1 < (uint64_t)(kSvcDataBytes - 1) ? cap - 1 : (uint64_t)(kSvcDataBytes - 1);
         if ((uint64_t)len < n) n = (uint64_t)len;
         svc_copy(out, s->data, n);
-        if (is_private_ptr(out) || elected()) out[n] = 0;
+        if (is_private_ptr(out)) GPUMPI_LANE_PRIVATE(out)[n] = 0;
+        else if (elected()) out[n] = 0;
         wave_sync();
     }
     svc_release(s);

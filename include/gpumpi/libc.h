@@ -104,6 +104,7 @@ __device__ inline void* copy(void* dst, const void* src, size_t n) {
     uint8_t* d = (uint8_t*)dst;
     const uint8_t* s = (const uint8_t*)src;
     if (gpumpi::rt::is_private_ptr(dst)) {
+        d = GPUMPI_LANE_PRIVATE(d);
         for (size_t i = 0; i < n; i++) d[i] = s[i];
     } else {
         for (size_t i = (size_t)gpumpi::rt::lane_id(); i < n; i += 64) d[i] = s[i];
@@ -114,6 +115,7 @@ __device__ inline void* copy(void* dst, const void* src, size_t n) {
 __device__ inline void* fill(void* dst, int c, size_t n) {
     uint8_t* d = (uint8_t*)dst;
     if (gpumpi::rt::is_private_ptr(dst)) {
+        d = GPUMPI_LANE_PRIVATE(d);
         for (size_t i = 0; i < n; i++) d[i] = (uint8_t)c;
     } else {
         for (size_t i = (size_t)gpumpi::rt::lane_id(); i < n; i += 64) d[i] = (uint8_t)c;

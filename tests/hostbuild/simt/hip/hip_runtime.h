@@ -64,6 +64,7 @@ void mem_step(const void* addr);
 void note_alloc(void* p, size_t n);
 void note_free(void* p);
 bool is_private(const void* p);
+void* lane_private(const void* p);
 bool is_shared(const void* p);
 uint64_t steady_counter();
 [[noreturn]] void fail(const char* what);
@@ -127,6 +128,7 @@ inline unsigned simt_alignbyte(unsigned hi, unsigned lo, unsigned r) {
 #define __builtin_amdgcn_alignbyte(hi, lo, r) simt_alignbyte((hi), (lo), (r))
 #define __builtin_readsteadycounter() ::simt::steady_counter()
 #define __builtin_amdgcn_is_private(p) ::simt::is_private((const void*)(p))
+#define GPUMPI_LANE_PRIVATE(p) ((decltype(p))::simt::lane_private((const void*)(p)))
 #define __builtin_amdgcn_is_shared(p) ::simt::is_shared((const void*)(p))
 
 #define __HIP_MEMORY_SCOPE_SINGLETHREAD 1

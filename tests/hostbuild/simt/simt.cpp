@@ -411,9 +411,22 @@ void block_barrier() {
     }
 }
 
+// Private (scratch) addresses as the GPU has them: one address names every
+// lane's own copy.  Here each lane's stack is its own region, all of one size
+// and with the same frames at the same offsets (the lanes run one program),
+// so an address in any lane's stack is private, and the calling lane's copy
+// of it lies at the same offset in its own stack.
+static char* g_stacks = nullptr;
+static char* g_stacks_end = nullptr;
 bool is_private(const void* p) {
     const char* c = (const char*)p;
-    return cur->stack_lo && c >= cur->stack_lo && c < cur->stack_lo + cur->stack_size;
+    if (cur->stack_lo && c >= cur->stack_lo && c < cur->stack_lo + cur->stack_size) return true;
+    return g_stacks && c >= g_stacks && c < g_stacks_end;
+}
+void* lane_private(const void* p) {
+    const char* c = (const char*)p;
+    if (!cur->stack_lo || !g_stacks || c < g_stacks || c >= g_stacks_end) return (void*)p;
+    return cur->stack_lo + (size_t)(c - g_stacks) % cur->stack_size;
 }
 bool is_shared(const void* p) {
     if (!cur->block) return false;
@@ -453,6 +466,8 @@ static int launch(const void* k, dim3 grid, dim3 block, void** args, size_t lds)
     char* stacks = (char*)mmap(nullptr, S * fibers.size(), PROT_READ | PROT_WRITE,
                                MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
     if (stacks == MAP_FAILED) return hipErrorOutOfMemory;
+    g_stacks = stacks;
+    g_stacks_end = stacks + S * fibers.size();
     for (unsigned b = 0; b < nb; b++) {
         Block& B = blocks[b];
         B.waves.resize(nt / 64);
@@ -532,6 +547,7 @@ static int launch(const void* k, dim3 grid, dim3 block, void** args, size_t lds)
             }
         }
     }
+    g_stacks = g_stacks_end = nullptr;
     munmap(stacks, S * fibers.size());
     for (Block& B : blocks) std::free(B.lds);
     cur = &main_fiber;

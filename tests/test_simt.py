@@ -400,10 +400,8 @@ NPB_IS = NPB
 @pytest.mark.skipif(not os.path.exists(os.path.join(NPB_IS, "IS", "is.c")), reason="the reference's NPB IS is absent")
 def test_sim_reference_npb_is_converts_and_verifies(tmp_path):
     # The reference's is.c, c_print_results.c and c_timers.c, unmodified,
-    # through the converter; class S on one rank passes NPB's verification.
-    # (More ranks reach full_verify's MPI_Irecv into a stack variable, which
-    # the simulation does not model: a lane's private address differs per
-    # lane here, while on the GPU one private address names every lane's own.)
+    # through the converter; class S on four ranks passes NPB's verification
+    # (full_verify receives into a stack variable with MPI_Irecv).
     import converter
     inc = _pkg_build().npb_params("is", "S")
     parts = []
@@ -415,7 +413,7 @@ def test_sim_reference_npb_is_converts_and_verifies(tmp_path):
                SIMT_CFLAGS=f"-I {NPB_IS}/IS -I {NPB_IS}/common -I {inc}")
     out = str(tmp_path / "out")
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), out, ":".join(parts)], check=True, env=env, timeout=1200)
-    rc, text = _run([os.path.join(out, "conv_is"), "---gpumpi", "-n", "1", "-b", "64", "-p", str(4 << 20)],
+    rc, text = _run([os.path.join(out, "conv_is"), "---gpumpi", "-n", "4", "-b", "64", "-p", str(4 << 20)],
                     timeout=1200)
     assert rc == 0, text[-3000:]
     assert "Verification    =               SUCCESSFUL" in text, text[-3000:]
