@@ -99,6 +99,12 @@ def build(jobs=8, force=False, verbose=True):
     ref_cpi = os.path.join(REFERENCE, "toolchain_tests", "pi", "cpi.c")
     if os.path.exists(ref_cpi):
         examples.append(([ref_cpi], os.path.join(LIB, "cpi"), []))
+    # the reference's own converter tests: globals (toolchain_tests/global_var)
+    # and MPI-IO (toolchain_tests/basicIO/basicIO.c)
+    for rel, exe in (("global_var/global_var.c", "global_var"), ("basicIO/basicIO.c", "basicIO")):
+        src = os.path.join(REFERENCE, "toolchain_tests", rel)
+        if os.path.exists(src):
+            examples.append(([src], os.path.join(LIB, exe), []))
     if os.path.exists(os.path.join(NPB, "IS", "is.c")) and shutil.which("gcc"):
         for cls in ("S", "B"):
             srcs = [os.path.join(NPB, "IS", "is.c"), os.path.join(NPB, "common", "c_print_results.c"),

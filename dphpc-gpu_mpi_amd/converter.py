@@ -97,15 +97,24 @@ class Locator:
 
     def __init__(self):
         self.file = None
+        self.spelled = None  # spelling location of the latest macro-expanded loc
 
     def see(self, loc):
+        """(file, offset, tokLen) of a location outside macro expansions,
+        else None; for a macro-expanded one, self.spelled is where its token
+        is written (a macro argument, or a #define body)."""
+        self.spelled = None
         if not isinstance(loc, dict):
             return None
+        spelled = None
         for key in ("spellingLoc", "expansionLoc"):
             if key in loc:
-                self.see(loc[key])
+                inner = self.see(loc[key])
+                if key == "spellingLoc":
+                    spelled = inner
         if "file" in loc:
             self.file = loc["file"]
+        self.spelled = spelled
         if "offset" in loc and "spellingLoc" not in loc:
             return (self.file, loc["offset"], loc.get("tokLen", 0))
         return None
@@ -117,11 +126,17 @@ def collect(ast, main_file, view_dir=None):
     lc = Locator()
     nodes = []
 
+    spellings = {}  # id(node) -> (begin, end) spelling locations of a macro-expanded node
+
     def visit(n, parents):
         loc = lc.see(n.get("loc"))
         rng = n.get("range", {})
         b = lc.see(rng.get("begin"))
+        bs = lc.spelled
         e = lc.see(rng.get("end"))
+        es = lc.spelled
+        if b is None and bs is not None and es is not None:
+            spellings[id(n)] = (bs, es)
         nodes.append((n, parents, loc, b, e))
         for c in n.get("inner", []):
             visit(c, parents + [n])
@@ -158,7 +173,14 @@ def collect(ast, main_file, view_dir=None):
     for n, parents, loc, b, e in nodes:
         k = n.get("kind")
         inside_function = any(p.get("kind") == "FunctionDecl" for p in parents)
-        if k == "DeclRefExpr" and in_main(b) and inside_function:
+        if k == "DeclRefExpr" and inside_function and b is None and id(n) in spellings:
+            # written inside a macro argument or a #define body of this file
+            bs, es = spellings[id(n)]
+            ref = n.get("referencedDecl", {})
+            if in_main(bs) and in_main(es) and bs[1] == es[1] and ref.get("id") in globals_:
+                edits.append((bs[1], 2, "insert", "__gpu_global("))
+                edits.append((es[1] + es[2], 0, "insert", ")"))
+        elif k == "DeclRefExpr" and in_main(b) and inside_function:
             ref = n.get("referencedDecl", {})
             if ref.get("id") in globals_:
                 edits.append((b[1], 2, "insert", "__gpu_global("))

@@ -83,3 +83,26 @@ def test_converted_reference_npb_dt(graph):
     text = p.stdout + p.stderr
     assert f"DT_{graph}.S L2 Norm = {gold['l2_norm']}" in text, text[-2000:]
     assert re.search(r"Verification\s+=\s+SUCCESSFUL", text)
+
+
+def test_converted_reference_global_var_test():
+    # toolchain_tests/global_var (the reference's converter test, 4 ranks as in
+    # its test.sh): per-rank globals; non-zero return and "error" lines on any
+    # mismatch.
+    exe = os.path.join(LIB, "global_var")
+    if not os.path.exists(exe):
+        pytest.skip("lib/global_var is not built (the reference's toolchain tests were not at hand)")
+    p = _run([exe, "---gpumpi", "-n", "4", "-b", "64"])
+    assert p.returncode == 0 and "error" not in p.stdout, p.stdout[-2000:] + p.stderr[-2000:]
+
+
+def test_converted_reference_basic_io_test(tmp_path):
+    # toolchain_tests/basicIO/basicIO.c: rank 0's greeting in test.out.
+    exe = os.path.join(LIB, "basicIO")
+    if not os.path.exists(exe):
+        pytest.skip("lib/basicIO is not built (the reference's toolchain tests were not at hand)")
+    print("[gpu] basicIO", flush=True)
+    p = subprocess.run([exe, "---gpumpi", "-n", "4", "-b", "64"], capture_output=True, text=True, timeout=120,
+                       cwd=str(tmp_path))
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert (tmp_path / "test.out").read_bytes() == b"hello world from GPUMPI IO"

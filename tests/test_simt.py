@@ -24,6 +24,7 @@ REF_CPI = "/root/reference/toolchain_tests/pi/cpi.c"
 
 
 SIM_MAX_CTX = 255
+REF_TOOLCHAIN = "/root/reference/toolchain_tests"
 NPB = "/root/reference/toolchain_tests/npb/NPB3.4.1/NPB3.4-MPI"
 
 
@@ -54,6 +55,11 @@ def sim():
         drivers.append(cpi)
     # Context ids limited to 3..255 here, so that id reuse is reached in a few
     # hundred communicator creations.
+    for rel, name in (("global_var/global_var.c", "conv_global_var.cpp"), ("basicIO/basicIO.c", "conv_basicio.cpp")):
+        src = os.path.join(REF_TOOLCHAIN, rel)
+        if os.path.exists(src):
+            converter.convert(src, os.path.join(OUT, name))
+            drivers.append(os.path.join(OUT, name))
     exit_prog = os.path.join(OUT, "conv_exit.cpp")
     converter.convert(os.path.join(HERE, "programs", "exit_early.c"), exit_prog)
     drivers.append(exit_prog)
@@ -469,3 +475,25 @@ def test_sim_converted_exit_ends_only_the_calling_rank(sim):
     assert "exit_early: size 4 sum 6" in out and out.count("rank leaving with 3") == 1
     printed = sorted(int(l.split(": ")[1]) for l in out.splitlines() if l.startswith("never printed by rank 1"))
     assert printed == [0, 2, 3], out[-3000:]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_TOOLCHAIN, "global_var", "global_var.c")),
+                    reason="the reference's toolchain tests are absent")
+def test_sim_reference_global_var_test_passes(sim):
+    # The reference's own converter test (toolchain_tests/global_var, run by
+    # its test.sh with 4 ranks): per-rank globals, a global written through a
+    # macro, several declarators in one declaration, static locals; it
+    # returns non-zero and prints "error ..." on any mismatch.
+    rc, out = _run([os.path.join(sim, "conv_global_var"), "---gpumpi", "-n", "4", "-b", "64"] + SMALL_HEAP)
+    assert rc == 0 and "error" not in out.replace("runtime error", ""), out[-3000:]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_TOOLCHAIN, "basicIO", "basicIO.c")),
+                    reason="the reference's toolchain tests are absent")
+def test_sim_reference_basic_io_test_writes_its_file(sim, tmp_path):
+    # toolchain_tests/basicIO/basicIO.c: rank 0 writes its greeting into
+    # test.out through MPI_File_write on a file every rank opened.
+    p = subprocess.run([os.path.join(sim, "conv_basicio"), "---gpumpi", "-n", "4", "-b", "64"] + SMALL_HEAP,
+                       capture_output=True, text=True, env=ENV, timeout=900, cwd=str(tmp_path))
+    assert p.returncode == 0, (p.stdout + p.stderr)[-3000:]
+    assert (tmp_path / "test.out").read_bytes() == b"hello world from GPUMPI IO"
