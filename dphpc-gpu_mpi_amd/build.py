@@ -141,6 +141,9 @@ def npb_params(bench, cls):
     11-15): config/make.def is its make.def.template with MPICC set to this
     package's gpumpicc, and its sys/setparams.c writes the header."""
     d = os.path.join(OBJ, "npb")
+    wd = os.path.join(d, f"{bench}_{cls}")
+    if os.path.exists(os.path.join(wd, "npbparams.h")):
+        return wd  # made already (it depends on the class alone)
     os.makedirs(os.path.join(d, "config"), exist_ok=True)
     with open(os.path.join(NPB, "config", "make.def.template")) as f:
         tmpl = f.read()
@@ -150,7 +153,6 @@ def npb_params(bench, cls):
     if not os.path.exists(sp):
         subprocess.run(["gcc", "-O2", "-o", sp, os.path.join(NPB, "sys", "setparams.c")], check=True,
                        capture_output=True)
-    wd = os.path.join(d, f"{bench}_{cls}")
     os.makedirs(wd, exist_ok=True)
     subprocess.run([sp, bench, cls], cwd=wd, check=True, stdout=subprocess.DEVNULL)
     return wd

@@ -36,6 +36,18 @@ def _pkg_build():
     return mod
 
 
+def _convert(converter, src, dst, incs=()):
+    """converter.convert into dst, leaving dst (and its timestamp) alone when
+    the text is the same, so that the incremental simulation build keeps its
+    object."""
+    tmp = dst + ".new"
+    converter.convert(src, tmp, list(incs))
+    if os.path.exists(dst) and open(dst).read() == open(tmp).read():
+        os.remove(tmp)
+    else:
+        os.replace(tmp, dst)
+
+
 @pytest.fixture(scope="module")
 def sim():
     # One build of every simulated program and driver: build.sh compiles all
@@ -47,21 +59,21 @@ def sim():
                                               "run_pic.cpp", "run_lifetime.cpp", "run_user_op.cpp",
                                               "run_p2p.cpp", "run_pic_pdecomp.cpp")]
     ring = os.path.join(OUT, "conv_ring.cpp")
-    converter.convert(os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
+    _convert(converter, os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
     drivers.append(ring)
     if os.path.exists(REF_CPI):
         cpi = os.path.join(OUT, "conv_cpi.cpp")
-        converter.convert(REF_CPI, cpi)
+        _convert(converter, REF_CPI, cpi)
         drivers.append(cpi)
     # Context ids limited to 3..255 here, so that id reuse is reached in a few
     # hundred communicator creations.
     for rel, name in (("global_var/global_var.c", "conv_global_var.cpp"), ("basicIO/basicIO.c", "conv_basicio.cpp")):
         src = os.path.join(REF_TOOLCHAIN, rel)
         if os.path.exists(src):
-            converter.convert(src, os.path.join(OUT, name))
+            _convert(converter, src, os.path.join(OUT, name))
             drivers.append(os.path.join(OUT, name))
     exit_prog = os.path.join(OUT, "conv_exit.cpp")
-    converter.convert(os.path.join(HERE, "programs", "exit_early.c"), exit_prog)
+    _convert(converter, os.path.join(HERE, "programs", "exit_early.c"), exit_prog)
     drivers.append(exit_prog)
     cflags = f"-DGPUMPI_MAX_CTX={SIM_MAX_CTX}"
     if os.path.exists(os.path.join(NPB, "DT", "dt.c")):
@@ -70,7 +82,7 @@ def sim():
         parts = []
         for src in ("DT/dt.c", "DT/DGraph.c", "common/c_print_results.c", "common/c_timers.c", "common/randdp.c"):
             dst = os.path.join(OUT, "conv_" + os.path.basename(src).replace(".c", ".cpp"))
-            converter.convert(os.path.join(NPB, src), dst, [inc])
+            _convert(converter, os.path.join(NPB, src), dst, [inc])
             parts.append(dst)
         drivers.append(":".join(parts))
         cflags += f" -I {NPB}/DT -I {NPB}/common -I {inc}"
