@@ -91,7 +91,7 @@ def build(jobs=8, force=False, verbose=True):
     # executables with the reference's `---gpumpi` command line.  lib/ring is
     # this package's example; lib/cpi is the reference's own pi program,
     # converted from its source when that is at hand.
-    # lib/is.S.x and lib/is.B.x are the reference's NPB IS (is.c with
+    # lib/is.S.x is the reference's NPB IS (is.c with
     # common/c_print_results.c and common/c_timers.c, unmodified), lib/dt.S.x
     # its NPB DT (dt.c, DGraph.c and the common files, unmodified).
     import gpumpicc
@@ -106,7 +106,7 @@ def build(jobs=8, force=False, verbose=True):
         if os.path.exists(src):
             examples.append(([src], os.path.join(LIB, exe), []))
     if os.path.exists(os.path.join(NPB, "IS", "is.c")) and shutil.which("gcc"):
-        for cls in ("S", "B"):
+        for cls in ("S",):  # the program runs as scalar code on each rank's wave 0: class S
             srcs = [os.path.join(NPB, "IS", "is.c"), os.path.join(NPB, "common", "c_print_results.c"),
                     os.path.join(NPB, "common", "c_timers.c")]
             examples.append((srcs, os.path.join(LIB, f"is.{cls}.x"), [npb_params("is", cls)]))

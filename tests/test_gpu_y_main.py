@@ -2,7 +2,7 @@
 of include/gpumpi/gpu_main.h (gpumpi_main) with the reference's `---gpumpi`
 command line (gpu_libs/gpu_main/main.cu:48-97), text through the host
 service channel.  lib/ring is dphpc-gpu_mpi_amd/examples/ring.c, lib/cpi the
-reference's toolchain_tests/pi/cpi.c, lib/is.S.x and lib/is.B.x its NPB IS,
+reference's toolchain_tests/pi/cpi.c, lib/is.S.x its NPB IS,
 lib/dt.S.x its NPB DT, all built by converter.py and gpumpicc.py
 (dphpc-gpu_mpi_amd/build.py)."""
 import json
@@ -51,8 +51,7 @@ def test_converted_reference_cpi(orc):
     assert abs(pi - host) <= 1e-13 * abs(host)
 
 
-@pytest.mark.parametrize("cls,P,heap", [("S", 1, 64 << 20), ("S", 4, 256 << 20), ("S", 16, 1 << 30),
-                                        ("B", 128, 16 << 30)])
+@pytest.mark.parametrize("cls,P,heap", [("S", 1, 64 << 20), ("S", 4, 256 << 20), ("S", 16, 1 << 30)])
 def test_converted_reference_npb_is(orc, cls, P, heap):
     # The reference's NPB IS (is.c, c_print_results.c, c_timers.c, unmodified)
     # converted into lib/is.<class>.x: NPB's own verification passes, as it
