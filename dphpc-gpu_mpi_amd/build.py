@@ -99,9 +99,11 @@ def build(jobs=8, force=False, verbose=True):
     ref_cpi = os.path.join(REFERENCE, "toolchain_tests", "pi", "cpi.c")
     if os.path.exists(ref_cpi):
         examples.append(([ref_cpi], os.path.join(LIB, "cpi"), []))
-    # the reference's own converter tests: globals (toolchain_tests/global_var)
-    # and MPI-IO (toolchain_tests/basicIO/basicIO.c)
-    for rel, exe in (("global_var/global_var.c", "global_var"), ("basicIO/basicIO.c", "basicIO")):
+    # the reference's own converter tests: globals (toolchain_tests/global_var),
+    # MPI-IO (basicIO/basicIO.c, basicIO/file_view1.c) and a contiguous
+    # datatype (datatype/contiguous.c)
+    for rel, exe in (("global_var/global_var.c", "global_var"), ("basicIO/basicIO.c", "basicIO"),
+                     ("datatype/contiguous.c", "contiguous"), ("basicIO/file_view1.c", "file_view1")):
         src = os.path.join(REFERENCE, "toolchain_tests", rel)
         if os.path.exists(src):
             examples.append(([src], os.path.join(LIB, exe), []))

@@ -289,6 +289,8 @@ __device__ inline int reduce_common(const void* sendbuf, void* recvbuf, int coun
     CollCtx cc;
     int err = coll_begin(comm, &cc);
     if (err) return err;
+    // A derived (contiguous) datatype reduces with a user-defined operator
+    // only: the predefined ones are defined on basic datatypes (op_allowed).
     const int es = type_size(dt);
     if (es <= 0) return GPUMPI_FAIL(MPI_ERR_TYPE, "invalid datatype in reduction");
     if (count < 0) return GPUMPI_FAIL(MPI_ERR_COUNT, "negative count in reduction");

@@ -274,7 +274,10 @@ __device__ inline int MPI_File_set_view(MPI_File fh, MPI_Offset disp, MPI_Dataty
                                         const char* datarep, MPI_Info) {
     using namespace gpumpi::rt;
     if (!file_valid(fh)) return MPI_ERR_FILE;
-    if (type_size(etype) <= 0 || filetype != etype) return MPI_ERR_TYPE;
+    // a dense view only: the filetype is etype or a contiguous run of its elements
+    if (type_size(etype) <= 0 || type_size(filetype) <= 0 || type_base(filetype) != type_base(etype) ||
+        type_size(filetype) % type_size(etype) != 0)
+        return MPI_ERR_TYPE;
     const char* native = "native";
     if (datarep) {
         int k = 0;

@@ -131,11 +131,35 @@ __device__ inline int MPI_Type_size(MPI_Datatype datatype, int* size) {
     *size = s;
     return MPI_SUCCESS;
 }
-__device__ inline int MPI_Type_contiguous(int, MPI_Datatype, MPI_Datatype*) {
-    return GPUMPI_FAIL(MPI_ERR_TYPE, "derived datatypes are not supported");
+// MPI_Type_contiguous: `count` copies of oldtype, itself basic or contiguous
+// (the reference's was NOT_IMPLEMENTED, datatypes.cu:10-15).  The handle
+// encodes the basic type and the element count, so it needs no storage,
+// commit or free work, and is the same value on every rank.
+__device__ inline int MPI_Type_contiguous(int count, MPI_Datatype oldtype, MPI_Datatype* newtype) {
+    using namespace gpumpi::rt;
+    if (!newtype) return GPUMPI_FAIL(MPI_ERR_ARG, "MPI_Type_contiguous: null newtype");
+    if (type_size(oldtype) <= 0) return GPUMPI_FAIL(MPI_ERR_TYPE, "MPI_Type_contiguous: invalid oldtype");
+    if (count < 1) return GPUMPI_FAIL(MPI_ERR_COUNT, "MPI_Type_contiguous: count must be at least 1");
+    const int64_t n = (int64_t)count * type_count(oldtype);
+    const int base = type_base(oldtype);
+    if (n > kContigMaxCount || n * basic_type_size(base) > 0x7fffffff)
+        return GPUMPI_FAIL(MPI_ERR_COUNT, "MPI_Type_contiguous: type too large");
+    *newtype = (MPI_Datatype)(GPUMPI_CONTIGUOUS_FLAG | (int)(n << kContigShift) | base);
+    return MPI_SUCCESS;
 }
-__device__ inline int MPI_Type_commit(MPI_Datatype*) {
-    return GPUMPI_FAIL(MPI_ERR_TYPE, "derived datatypes are not supported");
+__device__ inline int MPI_Type_commit(MPI_Datatype* datatype) {
+    if (!datatype || gpumpi::rt::type_size(*datatype) <= 0)
+        return GPUMPI_FAIL(MPI_ERR_TYPE, "MPI_Type_commit: invalid datatype");
+    return MPI_SUCCESS;
+}
+__device__ inline int MPI_Type_free(MPI_Datatype* datatype) {
+    if (!datatype || !gpumpi::rt::type_is_contig(*datatype))
+        return GPUMPI_FAIL(MPI_ERR_TYPE, "MPI_Type_free: not a derived datatype");
+    *datatype = MPI_DATATYPE_NULL;
+    return MPI_SUCCESS;
+}
+__device__ inline int MPI_Type_vector(int, int, int, MPI_Datatype, MPI_Datatype*) {
+    return GPUMPI_FAIL(MPI_ERR_TYPE, "strided derived datatypes (MPI_Type_vector) are not supported");
 }
 __device__ inline int MPI_Get_count(const MPI_Status* st, MPI_Datatype dt, int* count) {
     const int s = gpumpi::rt::type_size(dt);

@@ -13,7 +13,7 @@
 namespace gpumpi {
 namespace rt {
 
-__host__ __device__ inline int type_size(int dt) {
+__host__ __device__ inline int basic_type_size(int dt) {
     switch (dt) {
     case MPI_CHAR: case MPI_SIGNED_CHAR: case MPI_UNSIGNED_CHAR: case MPI_C_BOOL:
     case MPI_INT8_T: case MPI_UINT8_T: case MPI_BYTE: return 1;
@@ -26,6 +26,23 @@ __host__ __device__ inline int type_size(int dt) {
     case MPI_C_DOUBLE_COMPLEX: case MPI_C_LONG_DOUBLE_COMPLEX: return 16;
     default: return -1;
     }
+}
+
+// Contiguous derived types are their own handles:
+// GPUMPI_CONTIGUOUS_FLAG | count << 5 | basic type.
+constexpr int kContigShift = 5;
+constexpr int kContigMaxCount = (1 << 25) - 1;
+__host__ __device__ inline bool type_is_contig(int dt) { return dt > 0 && (dt & GPUMPI_CONTIGUOUS_FLAG) != 0; }
+__host__ __device__ inline int type_base(int dt) { return type_is_contig(dt) ? (dt & 31) : dt; }
+__host__ __device__ inline int type_count(int dt) {
+    return type_is_contig(dt) ? ((dt >> kContigShift) & kContigMaxCount) : 1;
+}
+// Bytes of one element of dt, or -1 for no valid datatype.
+__host__ __device__ inline int type_size(int dt) {
+    if (!type_is_contig(dt)) return basic_type_size(dt);
+    const int b = basic_type_size(dt & 31);
+    const int64_t n = (int64_t)type_count(dt) * b;
+    return b > 0 && n > 0 && n <= 0x7fffffff ? (int)n : -1;
 }
 
 enum TypeClass { TC_NONE, TC_INT, TC_FLOAT, TC_LOGICAL, TC_COMPLEX, TC_BYTE, TC_TEXT };

@@ -34,6 +34,10 @@
  *     does not truncate an existing file and MPI_MODE_EXCL fails only when
  *     the file exists; MPI_File_delete is local; read_at/write_at return the
  *     element count like read/write;
+ *   - MPI_Type_contiguous works (datatypes.cu:10-15 left it unimplemented):
+ *     a contiguous type of a basic type, usable wherever a datatype is (in
+ *     reductions with user-defined operators, as MPI defines the predefined
+ *     ones on basic datatypes only); MPI_Type_vector is still refused;
  *   - MPI_Comm_free is collective over the communicator's members: each
  *     completes its own requests on it first, and the last member out returns
  *     its memory and context id (the reference deleted on the last local
@@ -122,7 +126,11 @@ enum MPI_Datatype {
     MPI_UINT64_T, MPI_C_COMPLEX, MPI_C_FLOAT_COMPLEX, MPI_C_DOUBLE_COMPLEX,
     MPI_C_LONG_DOUBLE_COMPLEX, MPI_BYTE,
     GPUMPI_NUM_BASIC_TYPES,
-    MPI_DATATYPE_NULL = -1
+    MPI_DATATYPE_NULL = -1,
+    /* MPI_Type_contiguous handles: GPUMPI_CONTIGUOUS_FLAG | count << 5 | basic
+     * type (the same value on every rank, no table behind it) */
+    GPUMPI_CONTIGUOUS_FLAG = 1 << 30,
+    GPUMPI_DATATYPE_MAX = 0x7fffffff
 };
 
 /* ---- handles --------------------------------------------------------- */
@@ -202,6 +210,8 @@ __device__ double MPI_Wtick(void);
 __device__ int MPI_Type_size(MPI_Datatype datatype, int* size);
 __device__ int MPI_Type_contiguous(int count, MPI_Datatype oldtype, MPI_Datatype* newtype);
 __device__ int MPI_Type_commit(MPI_Datatype* datatype);
+__device__ int MPI_Type_free(MPI_Datatype* datatype);
+__device__ int MPI_Type_vector(int count, int blocklength, int stride, MPI_Datatype oldtype, MPI_Datatype* newtype);
 __device__ int MPI_Get_count(const MPI_Status* status, MPI_Datatype datatype, int* count);
 
 /* ---- operators: operators.cu:241-254 ----------------------------------- */

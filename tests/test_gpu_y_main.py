@@ -105,3 +105,26 @@ def test_converted_reference_basic_io_test(tmp_path):
                        cwd=str(tmp_path))
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     assert (tmp_path / "test.out").read_bytes() == b"hello world from GPUMPI IO"
+
+
+def test_converted_reference_contiguous_test():
+    # toolchain_tests/datatype/contiguous.c: MPI_Type_contiguous(100, MPI_CHAR).
+    exe = os.path.join(LIB, "contiguous")
+    if not os.path.exists(exe):
+        pytest.skip("lib/contiguous is not built (the reference's toolchain tests were not at hand)")
+    p = _run([exe, "---gpumpi", "-n", "2", "-b", "64"])
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+
+
+def test_converted_reference_file_view_test(tmp_path):
+    # toolchain_tests/basicIO/file_view1.c: 100 ints per rank at rank*400 B.
+    import numpy as np
+    exe = os.path.join(LIB, "file_view1")
+    if not os.path.exists(exe):
+        pytest.skip("lib/file_view1 is not built (the reference's toolchain tests were not at hand)")
+    print("[gpu] file_view1", flush=True)
+    p = subprocess.run([exe, "---gpumpi", "-n", "4", "-b", "64"], capture_output=True, text=True, timeout=120,
+                       cwd=str(tmp_path))
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    data = np.fromfile(str(tmp_path / "test_file_view1.out"), dtype=np.int32)
+    assert np.array_equal(data, np.tile(np.arange(100, dtype=np.int32), 4))

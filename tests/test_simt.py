@@ -67,11 +67,16 @@ def sim():
         drivers.append(cpi)
     # Context ids limited to 3..255 here, so that id reuse is reached in a few
     # hundred communicator creations.
-    for rel, name in (("global_var/global_var.c", "conv_global_var.cpp"), ("basicIO/basicIO.c", "conv_basicio.cpp")):
+    for rel, name in (("global_var/global_var.c", "conv_global_var.cpp"), ("basicIO/basicIO.c", "conv_basicio.cpp"),
+                      ("datatype/contiguous.c", "conv_contiguous.cpp"),
+                      ("basicIO/file_view1.c", "conv_file_view1.cpp")):
         src = os.path.join(REF_TOOLCHAIN, rel)
         if os.path.exists(src):
             _convert(converter, src, os.path.join(OUT, name))
             drivers.append(os.path.join(OUT, name))
+    derived = os.path.join(OUT, "conv_derived.cpp")
+    _convert(converter, os.path.join(HERE, "programs", "derived_types.c"), derived)
+    drivers.append(derived)
     exit_prog = os.path.join(OUT, "conv_exit.cpp")
     _convert(converter, os.path.join(HERE, "programs", "exit_early.c"), exit_prog)
     drivers.append(exit_prog)
@@ -509,3 +514,34 @@ def test_sim_reference_basic_io_test_writes_its_file(sim, tmp_path):
                        capture_output=True, text=True, env=ENV, timeout=900, cwd=str(tmp_path))
     assert p.returncode == 0, (p.stdout + p.stderr)[-3000:]
     assert (tmp_path / "test.out").read_bytes() == b"hello world from GPUMPI IO"
+
+
+@pytest.mark.parametrize("P", [2, 3, 4])
+def test_sim_converted_contiguous_datatypes(sim, P):
+    # tests/programs/derived_types.c (passes under host MPICH at 3 and 4
+    # ranks): contiguous types in sizes, nesting, Send/Recv into a stack
+    # buffer, Get_count, Bcast, and Allreduce/Reduce with user operators.
+    rc, out = _run([os.path.join(sim, "conv_derived"), "---gpumpi", "-n", str(P), "-b", "64"] + SMALL_HEAP)
+    assert rc == 0 and "derived_types: ok" in out, out[-3000:]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_TOOLCHAIN, "datatype", "contiguous.c")),
+                    reason="the reference's toolchain tests are absent")
+def test_sim_reference_contiguous_test(sim):
+    # toolchain_tests/datatype/contiguous.c: one MPI_Type_contiguous(100,
+    # MPI_CHAR) element from rank 0 to rank 1 (the reference left the type
+    # constructor NOT_IMPLEMENTED).
+    rc, out = _run([os.path.join(sim, "conv_contiguous"), "---gpumpi", "-n", "2", "-b", "64"] + SMALL_HEAP)
+    assert rc == 0, out[-3000:]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_TOOLCHAIN, "basicIO", "file_view1.c")),
+                    reason="the reference's toolchain tests are absent")
+def test_sim_reference_file_view_test(sim, tmp_path):
+    # toolchain_tests/basicIO/file_view1.c: each rank's 100 ints 0..99 at
+    # byte rank*400 through a view whose filetype is a contiguous type.
+    p = subprocess.run([os.path.join(sim, "conv_file_view1"), "---gpumpi", "-n", "4", "-b", "64"] + SMALL_HEAP,
+                       capture_output=True, text=True, env=ENV, timeout=900, cwd=str(tmp_path))
+    assert p.returncode == 0, (p.stdout + p.stderr)[-3000:]
+    data = np.fromfile(str(tmp_path / "test_file_view1.out"), dtype=np.int32)
+    assert np.array_equal(data, np.tile(np.arange(100, dtype=np.int32), 4))
