@@ -33,7 +33,9 @@ __device__ inline double value_f64(int rank, int i) {
     const double m = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
     return m * (double)(1ull << ((z >> 3) % 40));
 }
-__device__ inline long long value_i64(int rank, int i) { return (long long)((rank * 7919 + i * 104729) % 100003) - 50000; }
+__device__ inline long long value_i64(int rank, int i) {
+    return ((long long)rank * 7919 + (long long)i * 104729) % 100003 - 50000;
+}
 
 }  // namespace
 
@@ -54,6 +56,10 @@ GPUMPI_PROGRAM(user_op_reduce) {
     long long* xi = (long long*)gpumpi_malloc(sizeof(long long) * (uint64_t)n);
     long long* yi = (long long*)gpumpi_malloc(sizeof(long long) * (uint64_t)n);
     double* od = (double*)gpumpi_malloc(sizeof(double) * (uint64_t)n);
+    if (!xd || !yd || !zd || !xi || !yi || !od) {
+        MPI_Abort(MPI_COMM_WORLD, 98);
+        return 1;
+    }
     for (int i = lane; i < n; i += 64) {
         xd[i] = value_f64(rank, i);
         yd[i] = xd[i];

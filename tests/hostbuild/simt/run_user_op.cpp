@@ -1,6 +1,6 @@
 // CPU-simulated launch of the user-defined operator program
 // (progs/user_ops.hip):
-//   run_user_op P COUNT ROOT IN_PLACE THREADS
+//   run_user_op P COUNT ROOT IN_PLACE THREADS [HEAP_BYTES_PER_RANK]
 // Prints "ok R V" per rank and "sum S" (the sum of the asymmetric result).
 #include <cstdint>
 #include <cstdio>
@@ -30,7 +30,7 @@ int main(int argc, char** argv) {
     gpumpi_default_config(&cfg, P);
     cfg.threads_per_rank = threads;
     cfg.timeout_s = 0;
-    cfg.heap_bytes = (size_t)P * (1u << 20);
+    cfg.heap_bytes = (size_t)P * (argc > 6 ? std::strtoull(argv[6], nullptr, 10) : (1u << 20));
     gpumpi_result res;
     char name[] = "user_op_reduce";
     char* av[] = {name, nullptr};
