@@ -716,7 +716,9 @@ __device__ inline __attribute__((noinline)) int start_recv(void* buf, uint64_t b
 }
 
 // Status of a completed request (source translated to a rank of its comm).
+// Every lane reads the request before any lane goes on to release it.
 __device__ inline void fill_status(int idx, MPI_Status* st) {
+    wave_sync();
     if (!st) return;
     const Req* q = &reqs()[idx];
     int src = q->st_source;

@@ -1,5 +1,9 @@
 // CPU-simulated launch of the semantics scenarios (progs/semantics.hip):
-//   run_semantics P THREADS MASK
+//   run_semantics P THREADS MASK [HEAP_BYTES_PER_RANK]
+// The rank heaps together stay under the simulation's bulk-allocation size
+// (8 MiB, simt.cpp), so that the lanes' accesses to heap buffers are in
+// lockstep as on the GPU: the scenarios write a receive buffer from every
+// lane just before the receive fills it.
 // Prints one line per rank: "rank R: v0 v1 ..." (the rank's record).
 #include <cstdint>
 #include <cstdio>
@@ -27,6 +31,7 @@ int main(int argc, char** argv) {
     gpumpi_default_config(&cfg, P);
     cfg.threads_per_rank = threads;
     cfg.timeout_s = 0;
+    cfg.heap_bytes = (size_t)P * (argc > 4 ? std::strtoull(argv[4], nullptr, 10) : (512u << 10));
     gpumpi_result res;
     char name[] = "semantics";
     char* av[] = {name, nullptr};

@@ -57,7 +57,8 @@ def sim():
     os.makedirs(OUT, exist_ok=True)
     drivers = [os.path.join(SIM, d) for d in ("run_coll.cpp", "run_batched.cpp", "run_is.cpp", "run_io.cpp",
                                               "run_pic.cpp", "run_lifetime.cpp", "run_user_op.cpp",
-                                              "run_p2p.cpp", "run_pic_pdecomp.cpp")]
+                                              "run_p2p.cpp", "run_pic_pdecomp.cpp", "run_semantics.cpp",
+                                              "run_known_answer.cpp")]
     ring = os.path.join(OUT, "conv_ring.cpp")
     _convert(converter, os.path.join(HERE, "..", "dphpc-gpu_mpi_amd", "examples", "ring.c"), ring)
     drivers.append(ring)
@@ -92,7 +93,7 @@ def sim():
         drivers.append(":".join(parts))
         cflags += f" -I {NPB}/DT -I {NPB}/common -I {inc}"
     env = dict(os.environ, SIMT_PROGS="coll_bench.hip batched.hip npb_is.hip io.hip pic.hip pic_exact.hip lifetime.hip "
-               "user_ops.hip p2p_bench.hip pic_pdecomp.hip", SIMT_CFLAGS=cflags)
+               "user_ops.hip p2p_bench.hip pic_pdecomp.hip semantics.hip known_answer.hip", SIMT_CFLAGS=cflags)
     subprocess.run(["bash", os.path.join(SIM, "build.sh"), OUT] + drivers, check=True, env=env, timeout=1200)
     return OUT
 
@@ -545,3 +546,57 @@ def test_sim_reference_file_view_test(sim, tmp_path):
     assert p.returncode == 0, (p.stdout + p.stderr)[-3000:]
     data = np.fromfile(str(tmp_path / "test_file_view1.out"), dtype=np.int32)
     assert np.array_equal(data, np.tile(np.arange(100, dtype=np.int32), 4))
+
+
+# The reference's own known-answer tests (tests/cuda_mpi_test.cu,
+# collectives_test.cu, communicator_test.cu), the same cases and rank counts
+# as tests/test_gpu_known_answer.py.
+KNOWN_ANSWER = [("ka_single_int", 2, None), ("ka_array16", 2, None), ("ka_sendrecv_pm", 2, None),
+                ("ka_repeat_tags", 2, None), ("ka_network_flood", 3, None), ("ka_all_to_all_fifo", 10, 400),
+                ("ka_reduce_prod_i16", 5, None), ("ka_allreduce_double", 7, None), ("ka_allgather", 7, None),
+                ("ka_scatter", 5, None), ("ka_scatterv", 5, None), ("ka_comm_rank", 5, None),
+                ("ka_comm_dup_isolation", 2, None), ("ka_comm_split", 10, None)]
+
+
+@pytest.mark.parametrize("name,P,max_requests", KNOWN_ANSWER, ids=[c[0] for c in KNOWN_ANSWER])
+def test_sim_reference_known_answer_tests(sim, name, P, max_requests):
+    args = [os.path.join(sim, "run_known_answer"), name, str(P), "64"]
+    if max_requests:
+        args.append(str(max_requests))
+    rc, out = _run(args)
+    assert rc == 0 and "status=0" in out, out[-3000:]
+    oks = [int(l.split()[2]) for l in out.splitlines() if l.startswith("ok ")]
+    assert oks == [1] * P, out[-2000:]
+
+
+SEM_GOLDEN = os.path.join(HERE, "golden", "semantics_mpich.json")
+
+
+def _sim_semantics(sim, P, threads, mask, race=False):
+    rc, out = (_run_race if race else _run)([os.path.join(sim, "run_semantics"), str(P), str(threads), str(mask)])
+    assert rc == 0 and "status=0" in out, out[-3000:]
+    recs = {}
+    for line in out.splitlines():
+        if line.startswith("rank ") and ":" in line:
+            head, _, vals = line.partition(":")
+            recs[int(head.split()[1])] = [int(v) for v in vals.split()]
+    return [recs[r] for r in range(P)], out
+
+
+@pytest.mark.parametrize("P,threads", [(4, 64), (8, 256), (13, 64)])
+def test_sim_semantics_match_host_mpi(sim, P, threads):
+    # Every rank's record of the p2p / request / communicator scenarios equals
+    # host MPICH's (the fixture of tests/test_gpu_semantics.py).
+    import json
+    gold = json.load(open(SEM_GOLDEN))["runs"][str(P)]
+    got, out = _sim_semantics(sim, P, threads, gold["mask"])
+    for r in range(P):
+        assert got[r] == gold["records"][r], f"rank {r}: {got[r]} != {gold['records'][r]}"
+
+
+def test_sim_semantics_have_no_unordered_accesses(sim):
+    import json
+    gold = json.load(open(SEM_GOLDEN))["runs"]["8"]
+    got, out = _sim_semantics(sim, 8, 64, gold["mask"], race=True)
+    assert got == gold["records"]
+    assert "simt race:" not in out and "0 report(s)" in out, out[-3000:]
