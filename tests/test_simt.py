@@ -600,3 +600,13 @@ def test_sim_semantics_have_no_unordered_accesses(sim):
     got, out = _sim_semantics(sim, 8, 64, gold["mask"], race=True)
     assert got == gold["records"]
     assert "simt race:" not in out and "0 report(s)" in out, out[-3000:]
+
+
+@pytest.mark.parametrize("rendezvous", [0, 1])
+def test_sim_truncated_receive_is_reported(sim, rendezvous):
+    # A receive buffer smaller than the message (eager and rendezvous sizes):
+    # the launch ends with MPI_ERR_TRUNCATE recorded for the receiving rank,
+    # as tests/test_gpu_semantics.py asserts on the GPU.
+    rc, out = _run([os.path.join(sim, "run_semantics"), "truncate", str(rendezvous)])
+    assert rc == 0, out[-3000:]
+    assert "status=4 mpi_error=20 mpi_error_rank=0" in out, out[-3000:]
