@@ -5,6 +5,8 @@
 //   run_coll abort P THREADS                   (rank 1 aborts; the others keep
 //                                               calling collectives)
 //   run_coll wire FMT P N THREADS              (compressed-wire Allreduce)
+//   run_coll truncate THREADS                  (Alltoall send block larger
+//                                               than the receive block)
 //   run_coll race_selftest ORDERED             (the happens-before checker
 //                                               on a planted race, SIMT_RACE=1)
 #include "progs/registry.hpp"
@@ -142,6 +144,7 @@ static int blocks(int kind, int threads) {
     std::vector<uint8_t> sb(stride * P), rb(stride * P, 0);
     for (int r = 0; r < P; r++)
         for (int i = 0; i < 1024; i++) ((int32_t*)(sb.data() + r * stride))[i] = i + 1000 * r;
+    if (kind == 7 || kind == 8) rb = sb;  // in place: the receive buffer holds the blocks to send
     std::vector<uint64_t> words(64, 0);
     words[1] = (uint64_t)kind;
     words[2] = blk;
@@ -166,6 +169,16 @@ static int blocks(int kind, int threads) {
     case 3: for (int r = 0; r < P; r++) for (int i = 0; i < cnt[r]; i++) bad += got(r, i) != val(root, dsp[r] + i); break;
     case 4: for (int d = 0; d < P; d++) for (int r = 0; r < P; r++) for (int i = 0; i < blk; i++) bad += got(d, r * blk + i) != val(r, i); break;
     case 5: for (int d = 0; d < P; d++) for (int r = 0; r < P; r++) for (int i = 0; i < cnt[r]; i++) bad += got(d, dsp[r] + i) != val(r, i); break;
+    case 8:  // Alltoallv in place: ranks r and k exchange blk + (r + k) % 3 elements both ways
+        for (int r = 0; r < P; r++) {
+            int o = 0;
+            for (int src = 0; src < P; src++) {
+                int d = 0;
+                for (int k = 0; k < r; k++) d += blk + (src + k) % 3;
+                for (int i = 0; i < blk + (src + r) % 3; i++, o++) bad += got(r, o) != val(src, d + i);
+            }
+        }
+        break;
     default: for (int d = 0; d < P; d++) for (int s = 0; s < P; s++) for (int i = 0; i < blk; i++) bad += got(d, s * blk + i) != val(s, d * blk + i); break;
     }
     std::printf("blocks kind %d mismatches: %ld\n", kind, bad);
@@ -225,6 +238,26 @@ static int race_selftest(int ordered) {
     return launch("sim_race_selftest", 2, 64, words);
 }
 
+// coll_blocks kind 9: an Alltoall whose send blocks exceed the receive
+// blocks must end the launch with MPI_ERR_TRUNCATE.
+static int truncate_case(int threads) {
+    const int P = 6, blk = 5;
+    const uint64_t stride = 4 * 1024;
+    std::vector<uint8_t> sb(stride * P, 0), rb(stride * P, 0);
+    std::vector<uint64_t> words(64, 0);
+    words[1] = 9;
+    words[2] = blk;
+    words[4] = (uint64_t)(uintptr_t)sb.data();
+    words[5] = (uint64_t)(uintptr_t)rb.data();
+    words[6] = stride;
+    words[7] = stride;
+    int err = 0;
+    const int rc = launch("coll_blocks", P, threads, words, &err);
+    std::printf("truncate case: rc=%d mpi_error=%d (expect rc=%d, mpi_error=%d)\n", rc, err, GPUMPI_EDEVICE,
+                MPI_ERR_TRUNCATE);
+    return (rc == GPUMPI_EDEVICE && err == MPI_ERR_TRUNCATE) ? 0 : 1;
+}
+
 static int abort_case(int P, int threads) {
     std::vector<uint64_t> words(64, 0);
     int err = 0;
@@ -238,6 +271,7 @@ int main(int argc, char** argv) {
     if (argc >= 4 && !std::strcmp(argv[1], "alltoallv")) return alltoallv(argv[2], std::atoi(argv[3]));
     if (argc >= 4 && !std::strcmp(argv[1], "blocks")) return blocks(std::atoi(argv[2]), std::atoi(argv[3]));
     if (argc >= 3 && !std::strcmp(argv[1], "race_selftest")) return race_selftest(std::atoi(argv[2]));
+    if (argc >= 3 && !std::strcmp(argv[1], "truncate")) return truncate_case(std::atoi(argv[2]));
     if (argc >= 6 && !std::strcmp(argv[1], "wire"))
         return wire(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     std::fprintf(stderr, "usage: run_coll alltoallv COUNTS THREADS | blocks KIND THREADS | abort P THREADS\n");

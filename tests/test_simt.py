@@ -145,8 +145,9 @@ def test_sim_alltoallv_uneven(sim, tmp_path, P):
     assert rc == 0 and "alltoallv mismatches: 0" in out, out[-3000:]
 
 
-@pytest.mark.parametrize("kind", range(7))
+@pytest.mark.parametrize("kind", range(9))
 def test_sim_block_collectives(sim, kind):
+    # kinds 7 and 8: Alltoall and Alltoallv with MPI_IN_PLACE
     rc, out = _run([os.path.join(sim, "run_coll"), "blocks", str(kind), "128"])
     assert rc == 0 and f"blocks kind {kind} mismatches: 0" in out, out[-3000:]
 
@@ -610,3 +611,12 @@ def test_sim_truncated_receive_is_reported(sim, rendezvous):
     rc, out = _run([os.path.join(sim, "run_semantics"), "truncate", str(rendezvous)])
     assert rc == 0, out[-3000:]
     assert "status=4 mpi_error=20 mpi_error_rank=0" in out, out[-3000:]
+
+
+@pytest.mark.parametrize("threads", [64, 128])
+def test_sim_alltoall_truncation_is_an_error(sim, threads):
+    # An Alltoall whose send block exceeds the receive block ends the launch
+    # with MPI_ERR_TRUNCATE instead of cutting the block short (the case of
+    # tests/test_gpu_coll.py::test_alltoall_truncation_is_an_error).
+    rc, out = _run([os.path.join(sim, "run_coll"), "truncate", str(threads)])
+    assert rc == 0, out[-3000:]
