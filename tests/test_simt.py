@@ -620,3 +620,11 @@ def test_sim_alltoall_truncation_is_an_error(sim, threads):
     # tests/test_gpu_coll.py::test_alltoall_truncation_is_an_error).
     rc, out = _run([os.path.join(sim, "run_coll"), "truncate", str(threads)])
     assert rc == 0, out[-3000:]
+
+
+@pytest.mark.parametrize("name,P", [("ka_network_flood", 3), ("ka_comm_split", 10), ("ka_scatterv", 5),
+                                    ("ka_comm_dup_isolation", 2), ("ka_repeat_tags", 2)])
+def test_sim_known_answer_tests_have_no_unordered_accesses(sim, name, P):
+    rc, out = _run_race([os.path.join(sim, "run_known_answer"), name, str(P), "64"])
+    assert rc == 0 and "status=0" in out, out[-3000:]
+    assert "simt race:" not in out and "0 report(s)" in out, out[-3000:]
